@@ -1,0 +1,47 @@
+// kernels.hpp -- internal interface between the host dispatcher (api.cpp) and
+// the HIP translation units that hold the kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace sgemm_amd {
+
+enum Form { kFormF32 = 0, kFormBf16x3 = 1 };
+
+struct ConfigInfo {
+    int bm, bn, bk;  // workgroup tile
+    int wm, wn;      // waves along m and n
+    const char* name;
+};
+
+// Plain FP32 configs (sgemm_f32.hip).
+int f32_config_count();
+const ConfigInfo& f32_config(int id);
+hipError_t launch_f32(int id, const Problem& p, hipStream_t stream);
+
+// BF16x3 configs (sgemm_bf16x3.hip).
+int bf16x3_config_count();
+const ConfigInfo& bf16x3_config(int id);
+hipError_t launch_bf16x3(int id, const Problem& p, hipStream_t stream);
+
+// Naive one-thread-per-output kernel (sgemm_f32.hip).
+hipError_t launch_basic(const Problem& p, hipStream_t stream);
+
+// C := beta*C for the k == 0 / alpha == 0 shortcut (sgemm_f32.hip).
+hipError_t launch_scale_c(const Problem& p, hipStream_t stream);
+
+inline int config_count(int form) {
+    return form == kFormF32 ? f32_config_count() : form == kFormBf16x3 ? bf16x3_config_count() : 0;
+}
+inline const ConfigInfo& config_info(int form, int id) {
+    return form == kFormF32 ? f32_config(id) : bf16x3_config(id);
+}
+
+// True when every float4 access of the vector path is 16-byte aligned.
+inline bool vec4_ok(const Problem& p) {
+    auto al = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
+    return al(p.A) && al(p.B) && al(p.C) && (p.lda % 4 == 0) && (p.ldb % 4 == 0) &&
+           (p.ldc % 4 == 0);
+}
+
+}  // namespace sgemm_amd

@@ -1,0 +1,245 @@
+"""GPU parity of the plain FP32 path (sgemm / sgemm_ex / sgemm_rm / sgemm_basic)
+against the FP64 oracle, through the C ABI.
+
+Tolerance (plain FP32, any summation order, with or without FMA):
+    |C - C_ref|_ij <= 2 * gamma_{k+2} * (|alpha| (|A||B|)_ij + |beta| |C0|_ij)
+gamma_k = k u / (1 - k u), u = 2^-24.  Integer-valued inputs (|x| <= 4,
+k <= 8192) make every partial sum exact, so those cases must be bit-exact.
+
+Mirrors the reference's test.cu (sgemm vs the naive kernel, alpha=1.5,
+beta=0.5, test.cu:71-83) and widens it: non-square shapes, ld > dim, beta=0
+with a NaN-poisoned C, k=0, invalid arguments, unaligned operands.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import sgemm_amd as sa
+from gpu_util import Case, assert_ok, check, compact, download, log, run_form, upload
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gemm_cases.json")
+N_F32_CONFIGS = 4
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with open(GOLDEN) as f:
+        return json.load(f)["cases"]
+
+
+def test_config_table():
+    cfgs = sa.configs(sa.FORM_F32)
+    assert len(cfgs) == N_F32_CONFIGS
+    assert all(c["bm"] % 16 == 0 and c["bn"] % 16 == 0 for c in cfgs)
+
+
+@pytest.mark.parametrize("config", range(N_F32_CONFIGS))
+def test_golden_cases(golden, config):
+    for g in golden:
+        case = Case(g["m"], g["n"], g["k"], g["lda"], g["ldb"], g["ldc"], alpha=g["alpha"],
+                    beta=g["beta"])
+        case.A = np.array(g["A"], np.float32)
+        case.B = np.array(g["B"], np.float32)
+        case.C0 = np.array(g["C0"], np.float32)
+        out = run_form(case, sa.FORM_F32, config)
+        got = compact(out, g["m"], g["n"], g["ldc"])
+        if g["kind"] == "int":
+            assert np.array_equal(got, np.array(g["expected_f32_rounded"], np.float32)), g["name"]
+        st = check(case, out, orc.fp32_tol_mult(g["k"]))
+        assert_ok(st, f"{g['name']} config {config}")
+
+
+def test_basic_kernel_bit_exact_golden(golden):
+    """sgemm_basic follows kernel_basic's exact operation order."""
+    for g in golden:
+        dA, dB, dC = upload(np.array(g["A"])), upload(np.array(g["B"])), upload(np.array(g["C0"]))
+        log(f"sgemm_basic golden {g['name']}")
+        assert sa.sgemm_basic(g["m"], g["n"], g["k"], g["alpha"], dA, g["lda"], dB, g["ldb"],
+                              g["beta"], dC, g["ldc"]) == 0
+        got = compact(download(dC), g["m"], g["n"], g["ldc"])
+        assert np.array_equal(got, np.array(g["expected_basic_f32"], np.float32)), g["name"]
+
+
+def test_basic_kernel_bit_exact_vs_restatement():
+    c = Case(150, 97, 301, lda=160, ldb=311, ldc=151, seed=7)
+    dA, dB, dC = c.device()
+    log("sgemm_basic 150x97x301")
+    assert sa.sgemm_basic(c.m, c.n, c.k, c.alpha, dA, c.lda, dB, c.ldb, c.beta, dC, c.ldc) == 0
+    want = c.C0.copy()
+    orc.sgemm_basic_cm(c.m, c.n, c.k, c.alpha, c.A, c.lda, c.B, c.ldb, c.beta, want, c.ldc)
+    assert np.array_equal(download(dC), want)
+
+
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (1.5, 0.5)])
+def test_config1_1024(alpha, beta):
+    c = Case(1024, 1024, 1024, seed=101, alpha=alpha, beta=beta)
+    out = run_form(c, sa.FORM_F32)
+    st = check(c, out, orc.fp32_tol_mult(c.k))
+    assert_ok(st, f"1024^3 alpha={alpha} beta={beta}")
+
+
+@pytest.mark.parametrize("config", range(N_F32_CONFIGS))
+def test_integer_kat_bit_exact(config):
+    """Integer inputs in [-4, 4]: every partial sum is exact, so any summation
+    order must give exactly the FP64 result (alpha=1.5, beta=0.5 keep it exact)."""
+    for (m, n, k) in [(300, 257, 129), (129, 130, 1000), (64, 64, 8192)]:
+        c = Case(m, n, k, lda=m + 3, ldb=k + 1, ldc=m + 5, kind="int", seed=m + n + k)
+        out = run_form(c, sa.FORM_F32, config)
+        ref, _ = c.reference()
+        assert np.array_equal(compact(out, m, n, c.ldc).astype(np.float64), ref), (m, n, k)
+
+
+def _edge_shapes():
+    shapes = [(s, s, s) for s in range(1, 34)]
+    for bm in (64, 128, 256):
+        for d in (-1, 0, 1):
+            shapes.append((bm + d, 65 + d, 33 + d))
+            shapes.append((17, bm + d, 47))
+    rng = np.random.default_rng(2024)
+    for _ in range(60):
+        shapes.append(tuple(int(v) for v in rng.integers(1, 161, size=3)))
+    return shapes
+
+
+@pytest.mark.parametrize("config", range(N_F32_CONFIGS))
+def test_edge_sweep(config):
+    for (m, n, k) in _edge_shapes():
+        c = Case(m, n, k, lda=m + (m % 3), ldb=k + (k % 2), ldc=m + 1, seed=m * 7 + n * 3 + k)
+        out = run_form(c, sa.FORM_F32, config)
+        st = check(c, out, orc.fp32_tol_mult(k))
+        assert_ok(st, f"edge {(m, n, k)} config {config}")
+
+
+def test_beta_zero_does_not_read_nan_c():
+    for config in range(N_F32_CONFIGS):
+        c = Case(190, 77, 45, seed=3, alpha=1.25, beta=0.0, c_fill=np.nan)
+        out = run_form(c, sa.FORM_F32, config)
+        got = compact(out, c.m, c.n, c.ldc)
+        assert not np.isnan(got).any()
+        st = check(c, out, orc.fp32_tol_mult(c.k))
+        assert_ok(st, f"beta=0 NaN C config {config}")
+
+
+def test_k_zero_and_alpha_zero_scale_c():
+    for (k, alpha, beta) in [(0, 1.5, 0.5), (33, 0.0, 0.5), (0, 1.0, 0.0)]:
+        c = Case(70, 20, k, seed=5, alpha=alpha, beta=beta)
+        out = run_form(c, sa.FORM_F32)
+        want = np.float32(beta) * compact(c.C0, 70, 20, c.ldc) if beta != 0 else 0.0
+        assert np.array_equal(compact(out, 70, 20, c.ldc), np.broadcast_to(want, (1400,)))
+
+
+def test_empty_is_noop():
+    C = upload(np.arange(10, dtype=np.float32))
+    for (m, n) in [(0, 5), (5, 0), (0, 0)]:
+        assert sa.sgemm(m, n, 3, 1.0, C, max(1, m), C, 3, 0.0, C, max(1, m)) == 0
+    assert np.array_equal(download(C), np.arange(10, dtype=np.float32))
+
+
+def test_invalid_arguments_launch_nothing():
+    C = upload(np.full(16, 3.0, np.float32))
+    A = upload(np.ones(16, np.float32))
+    bad = [
+        (-1, 4, 4, 4, 4, 4), (4, -1, 4, 4, 4, 4), (4, 4, -1, 4, 4, 4),
+        (4, 4, 4, 3, 4, 4), (4, 4, 4, 4, 3, 4), (4, 4, 4, 4, 4, 3),
+    ]
+    for (m, n, k, lda, ldb, ldc) in bad:
+        assert sa.sgemm(m, n, k, 1.0, A, lda, A, ldb, 0.0, C, ldc) == 1
+        assert sa.sgemm_ex(m, n, k, 1.0, A, lda, A, ldb, 0.0, C, ldc, check=False) == 1
+    assert sa.sgemm(4, 4, 4, 1.0, None, 4, A, 4, 0.0, C, 4) == 1
+    assert sa.sgemm(4, 4, 4, 1.0, A, 4, A, 4, 0.0, None, 4) == 1
+    assert sa.run_config(0, 99, 4, 4, 4, 1.0, A, 4, A, 4, 0.0, C, 4, check=False) == 2
+    assert np.array_equal(download(C), np.full(16, 3.0, np.float32))
+
+
+def test_rowmajor_shim_matches_reference_semantics():
+    """sgemm_rm keeps the reference's row-major meaning (src/sgemm.cuh:31-33)."""
+    m, n, k = 300, 170, 90
+    A = orc.fill("uniform", m * k, 11)   # row-major m x k, lda = k
+    B = orc.fill("uniform", k * n, 12)   # row-major k x n, ldb = n
+    C = orc.fill("uniform", m * n, 13)   # row-major m x n, ldc = n
+    dA, dB, dC = upload(A), upload(B), upload(C)
+    log("sgemm_rm 300x170x90")
+    assert sa.sgemm_rm(m, n, k, 1.5, dA, k, dB, n, 0.5, dC, n) == 0
+    got = download(dC).reshape(m, n)
+    want = 1.5 * (A.reshape(m, k).astype(np.float64) @ B.reshape(k, n)) + 0.5 * C.reshape(m, n)
+    scale = 1.5 * (np.abs(A.reshape(m, k)).astype(np.float64) @ np.abs(B.reshape(k, n))) \
+        + 0.5 * np.abs(C.reshape(m, n))
+    assert np.all(np.abs(got - want) <= orc.fp32_tol_mult(k) * scale)
+
+
+def test_unaligned_operands_take_scalar_path():
+    """Odd leading dimensions and 4-byte-offset pointers (no 16-byte vectors)."""
+    import torch
+    m, n, k = 131, 67, 77
+    c = Case(m, n, k, lda=m + 2, ldb=k + 1, ldc=m + 2, seed=17)
+    for config in range(N_F32_CONFIGS):
+        dA = torch.zeros(c.A.size + 1, dtype=torch.float32, device="cuda")
+        dB = torch.zeros(c.B.size + 1, dtype=torch.float32, device="cuda")
+        dC = torch.zeros(c.C0.size + 1, dtype=torch.float32, device="cuda")
+        dA[1:] = upload(c.A)
+        dB[1:] = upload(c.B)
+        dC[1:] = upload(c.C0)
+        base = 4  # one float
+        log(f"unaligned config {config}")
+        sa.run_config(0, config, m, n, k, c.alpha, dA.data_ptr() + base, c.lda,
+                      dB.data_ptr() + base, c.ldb, c.beta, dC.data_ptr() + base, c.ldc)
+        out = download(dC)[1:]
+        assert_ok(check(c, out, orc.fp32_tol_mult(k)), f"unaligned config {config}")
+
+
+def test_stream_argument():
+    import torch
+    s = torch.cuda.Stream()
+    c = Case(513, 257, 129, seed=23)
+    out = run_form(c, sa.FORM_F32, stream=s)
+    assert_ok(check(c, out, orc.fp32_tol_mult(c.k)), "explicit stream")
+
+
+# ---------------------------------------------------------------- full sizes
+def _sampled_check(c: Case, out: np.ndarray, n_samples=4096, seed=0):
+    rng = np.random.default_rng(seed)
+    ii = rng.integers(0, c.m, n_samples).astype(np.int32)
+    jj = rng.integers(0, c.n, n_samples).astype(np.int32)
+    # always include the four corners and the last row/column edges
+    ii = np.concatenate([ii, [0, c.m - 1, 0, c.m - 1]]).astype(np.int32)
+    jj = np.concatenate([jj, [0, 0, c.n - 1, c.n - 1]]).astype(np.int32)
+    ref, scale = orc.dgemm_entries(c.k, c.alpha, c.A, c.lda, c.B, c.ldb, c.beta, c.C0, c.ldc, ii, jj)
+    got = out[ii + jj.astype(np.int64) * c.ldc].astype(np.float64)
+    tol = orc.fp32_tol_mult(c.k) * scale
+    err = np.abs(got - ref)
+    assert np.all(np.isfinite(got)) and np.all(err <= tol), \
+        f"{(c.m, c.n, c.k)}: max err/tol {float(np.max(err / tol))}"
+    return float(np.max(err / np.maximum(scale, 1e-300)))
+
+
+@pytest.mark.parametrize("m,n,k", [(1000, 1000, 1000), (2048, 2048, 2048), (4097, 3071, 2049),
+                                   (8192, 8192, 127), (8192, 8192, 8192)])
+def test_sweep_shapes_sampled_fp64_and_full_vs_naive(m, n, k):
+    """Baseline sweep shapes: sampled FP64 entries, plus every entry against the
+    independent naive kernel (itself bit-exact to the oracle's restatement)."""
+    import torch
+    c = Case(m, n, k, seed=m + 2 * n + 3 * k)
+    dA, dB, dC = c.device()
+    dC2 = dC.clone()
+    log(f"sweep {(m, n, k)} sgemm")
+    assert sa.sgemm(m, n, k, c.alpha, dA, c.lda, dB, c.ldb, c.beta, dC, c.ldc) == 0
+    log(f"sweep {(m, n, k)} sgemm_basic")
+    assert sa.sgemm_basic(m, n, k, c.alpha, dA, c.lda, dB, c.ldb, c.beta, dC2, c.ldc) == 0
+    torch.cuda.synchronize()
+    out = download(dC)
+    _sampled_check(c, out)
+    # naive kernel is within tol of truth, the fast one too: differ by <= 2 tol
+    # scale = |alpha| |A||B| + |beta| |C0|, by the naive kernel (not the one under test)
+    absA = dA.abs()
+    absB = dB.abs()
+    scale = torch.zeros_like(dC)
+    assert sa.sgemm_basic(m, n, k, abs(c.alpha), absA, c.lda, absB, c.ldb, 0.0, scale, c.ldc) == 0
+    scale += abs(c.beta) * dC2.new_tensor(c.C0).abs()
+    diff = (dC - dC2).abs()
+    bound = 2 * orc.fp32_tol_mult(k) * 1.001 * scale
+    assert bool(torch.all(diff <= bound)), float((diff / bound).max())
