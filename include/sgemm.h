@@ -79,6 +79,9 @@ void sgemm_rm(int m, int n, int k, const float* alpha, float* A, int lda, float*
  * cleared).  The reference reports nothing (src/sgemm.cuh:19-90). */
 int sgemm_get_last_error(void);
 const char* sgemm_status_string(int status);
+/* Text of the last HIP runtime error behind a SGEMM_STATUS_HIP_ERROR on this
+ * host thread. */
+const char* sgemm_hip_error_string(void);
 
 /* ------------------------------------------------------------------------ */
 /* Status-returning, stream-aware forms (no reference counterpart; the       */
@@ -106,6 +109,142 @@ int sgemm_config_count(int form);
 /* Writes "BMxBNxBK/WMxWN" style text; returns the tile sizes. */
 int sgemm_config_info(int form, int config_id, int* bm, int* bn, int* bk, char* name,
                       int name_len);
+
+/* ------------------------------------------------------------------------ */
+/* Grouped GEMM: many independent problems in one persistent launch (no     */
+/* reference counterpart).                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* One problem of a group: column-major, device operands, scalars by value.
+ * Same argument rules as sgemm_ex. */
+typedef struct {
+    int m, n, k;
+    float alpha, beta;
+    const float* A;
+    int lda;
+    const float* B;
+    int ldb;
+    float* C;
+    int ldc;
+} sgemm_problem;
+
+typedef struct sgemm_grouped_plan_s* sgemm_grouped_plan;
+
+/* A plan holds device descriptors for up to `capacity` problems of one form. */
+int sgemm_grouped_plan_create(sgemm_grouped_plan* plan, int capacity, int form);
+/* Validate, pick each problem's tile config from the autotune table, order the
+ * tiles (most expensive first) and upload the descriptors on `stream`.  The
+ * problems array is copied; it may be reused when this returns. */
+int sgemm_grouped_plan_set(sgemm_grouped_plan plan, int count, const sgemm_problem* problems,
+                           sgemm_stream_t stream);
+/* As sgemm_grouped_plan_set with an explicit config id per problem. */
+int sgemm_grouped_plan_set_configs(sgemm_grouped_plan plan, int count,
+                                   const sgemm_problem* problems, const int* config_ids,
+                                   sgemm_stream_t stream);
+/* One persistent kernel launch computing every problem of the plan. */
+int sgemm_grouped_plan_run(sgemm_grouped_plan plan, sgemm_stream_t stream);
+/* Capture the launch into a hipGraph (stream capture on `stream`) ... */
+int sgemm_grouped_plan_capture(sgemm_grouped_plan plan, sgemm_stream_t stream);
+/* ... and launch that graph.  The graph reads the plan's current descriptors,
+ * so a later sgemm_grouped_plan_set (same plan) needs no new capture. */
+int sgemm_grouped_plan_replay(sgemm_grouped_plan plan, sgemm_stream_t stream);
+int sgemm_grouped_plan_info(sgemm_grouped_plan plan, int* count, int* total_tiles, int* grid,
+                            double* flops);
+/* Waits for the device, then frees the plan. */
+int sgemm_grouped_plan_destroy(sgemm_grouped_plan plan);
+/* One-shot: plan, run on `stream`, wait for it, free. */
+int sgemm_grouped(int count, const sgemm_problem* problems, int form, sgemm_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Autotuner (no reference counterpart): one tile config per (form, grouped, */
+/* size bucket of m, n, k), bucket(x) = ceil(log2 x) capped to [0, 15].      */
+/* Selection is a pure function of the shape and the loaded table.  A table  */
+/* named by the environment variable SGEMM_AMD_TUNE_TABLE loads on first use.*/
+/* ------------------------------------------------------------------------ */
+
+/* Time every candidate config on an (m, n, k) problem (`iters` launches each,
+ * after one warm-up) and record the fastest for that shape's bucket.
+ * Allocates its own operands; not for hot paths. */
+int sgemm_autotune_bucket(int form, int grouped, int m, int n, int k, int iters,
+                          sgemm_stream_t stream, int* best_config, float* best_tflops);
+/* Config the library will use for this shape; *from_table = 1 if a table
+ * entry (not the built-in heuristic) decided. */
+int sgemm_autotune_lookup(int form, int grouped, int m, int n, int k, int* config_id,
+                          int* from_table);
+/* Set one bucket's config by hand. */
+int sgemm_autotune_set(int form, int grouped, int m, int n, int k, int config_id);
+int sgemm_autotune_clear(void);
+int sgemm_autotune_size(void);
+/* Persist / restore the table (JSON, one entry per line). */
+int sgemm_autotune_save(const char* path);
+int sgemm_autotune_load(const char* path);
+
+/* ------------------------------------------------------------------------ */
+/* Serving loop: continuous batching of arriving GEMM requests into grouped  */
+/* launches, admitted against a fixed HBM budget (no reference counterpart). */
+/* ------------------------------------------------------------------------ */
+
+enum { SGEMM_EXECUTOR_GPU = 0, SGEMM_EXECUTOR_MOCK = 1 };
+enum { SGEMM_OPERANDS_DEVICE = 0, SGEMM_OPERANDS_HOST = 1 };
+
+typedef struct {
+    size_t hbm_budget_bytes; /* admission budget; with host operands also the pool size */
+    int max_batch;           /* requests per grouped launch */
+    int num_slots;           /* grouped launches in flight at once (one stream each), 1..16 */
+    int form;                /* SGEMM_FORM_F32 or SGEMM_FORM_BF16X3 */
+    int executor;            /* SGEMM_EXECUTOR_GPU, or _MOCK (no device: a batch completes
+                                after flops / mock_tflops seconds) */
+    double mock_tflops;
+    double mock_launch_us;
+} sgemm_server_config;
+
+/* One request.  operands == SGEMM_OPERANDS_DEVICE: A, B, C are device memory
+ * the caller keeps alive until the request completes.  SGEMM_OPERANDS_HOST:
+ * host memory; the server copies A, B (and C if beta != 0) into its pool on
+ * admission and C back on completion. */
+typedef struct {
+    int m, n, k;
+    float alpha, beta;
+    const float* A;
+    int lda;
+    const float* B;
+    int ldb;
+    float* C;
+    int ldc;
+    int operands;
+} sgemm_request;
+
+typedef struct {
+    long long submitted, completed, rejected, failed;
+    long long batches;
+    long long measured;   /* completed requests the latency figures cover */
+    double mean_batch;    /* requests per grouped launch */
+    double p50_ms, p99_ms, mean_ms, max_ms; /* completion - submission */
+    double mean_queue_ms;                    /* admission - submission */
+    double window_s;      /* first submission .. last completion */
+    double tflops;        /* sum of 2mnk over that window */
+    size_t peak_bytes;    /* most budget bytes in use at once */
+    size_t budget_bytes;
+} sgemm_server_stats;
+
+typedef struct sgemm_server_s* sgemm_server;
+
+int sgemm_server_create(sgemm_server* srv, const sgemm_server_config* cfg);
+/* Queue a request; *id names it.  A request whose footprint
+ * 4*(mk + kn + mn) bytes exceeds the budget is rejected with
+ * SGEMM_STATUS_BUDGET_EXCEEDED. */
+int sgemm_server_submit(sgemm_server srv, const sgemm_request* req, uint64_t* id);
+/* SGEMM_STATUS_NOT_READY while queued or in flight, then its final status. */
+int sgemm_server_poll(sgemm_server srv, uint64_t id);
+int sgemm_server_wait(sgemm_server srv, uint64_t id, double timeout_s);
+/* Wait until every submitted request has finished. */
+int sgemm_server_drain(sgemm_server srv, double timeout_s);
+int sgemm_server_latency(sgemm_server srv, uint64_t id, double* total_ms, double* queue_ms);
+int sgemm_server_get_stats(sgemm_server srv, sgemm_server_stats* out);
+/* Latency statistics then cover requests submitted after this call. */
+int sgemm_server_reset_stats(sgemm_server srv);
+/* Finishes in-flight launches, then frees everything. */
+int sgemm_server_destroy(sgemm_server srv);
 
 /* Library version string. */
 const char* sgemm_version(void);

@@ -19,6 +19,7 @@ using namespace sgemm_amd;
 namespace {
 
 thread_local int g_last_error = SGEMM_STATUS_SUCCESS;
+thread_local hipError_t g_last_hip_error = hipSuccess;
 
 int record(int status) {
     g_last_error = status;
@@ -56,7 +57,11 @@ Problem make_problem(int m, int n, int k, float alpha, const float* A, int lda, 
     return p;
 }
 
-int hip_status(hipError_t e) { return e == hipSuccess ? SGEMM_STATUS_SUCCESS : SGEMM_STATUS_HIP_ERROR; }
+int hip_status(hipError_t e) {
+    if (e == hipSuccess) return SGEMM_STATUS_SUCCESS;
+    g_last_hip_error = e;
+    return SGEMM_STATUS_HIP_ERROR;
+}
 
 // Shared body of every single-problem entry point.  config_id < 0 selects the
 // config from the autotune table.
@@ -159,6 +164,10 @@ int sgemm_config_info(int form, int config_id, int* bm, int* bn, int* bk, char* 
     if (bk) *bk = c.bk;
     if (name && name_len > 0) std::snprintf(name, (size_t)name_len, "%s", c.name);
     return SGEMM_STATUS_SUCCESS;
+}
+
+const char* sgemm_hip_error_string(void) {
+    return g_last_hip_error == hipSuccess ? "no HIP error" : hipGetErrorString(g_last_hip_error);
 }
 
 const char* sgemm_version(void) { return "sgemm_amd 0.1 (gfx950)"; }

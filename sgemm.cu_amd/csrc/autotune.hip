@@ -1,0 +1,299 @@
+// autotune.hip -- the project's autotuner: picks a tile config per
+// (form, size bucket of m, n, k) by timing the candidates on the device, keeps
+// the winners in a table, and saves / loads that table as JSON.
+//
+// Selection is a pure function of (form, grouped, m, n, k) and the loaded
+// table: the same shape always gets the same config, which is what makes a
+// problem's bits independent of what else runs beside it (grouped launches,
+// the serving loop).  Without a table entry the built-in heuristic applies.
+// A table named by the environment variable SGEMM_AMD_TUNE_TABLE is loaded on
+// first use.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sgemm.h"
+#include "autotune.hpp"
+#include "kernels.hpp"
+
+namespace sgemm_amd {
+
+namespace {
+
+struct Entry {
+    int config;
+    float tflops;
+};
+
+struct Table {
+    std::mutex mu;
+    std::map<int, Entry> entries;  // key(form, grouped, mb, nb, kb)
+    bool env_loaded = false;
+};
+
+Table& table() {
+    static Table t;
+    return t;
+}
+
+int key(int form, int grouped, int mb, int nb, int kb) {
+    return (((form * 2 + grouped) * 16 + mb) * 16 + nb) * 16 + kb;
+}
+
+void unkey(int key, int& form, int& grouped, int& mb, int& nb, int& kb) {
+    kb = key % 16;
+    key /= 16;
+    nb = key % 16;
+    key /= 16;
+    mb = key % 16;
+    key /= 16;
+    grouped = key % 2;
+    form = key / 2;
+}
+
+bool grouped_capable(int form, int cfg) {
+    return form == kFormF32 ? (cfg >= 0 && cfg <= 2) : (cfg >= 0 && cfg <= 1);
+}
+
+int load_locked(Table& t, const char* path);
+
+void ensure_env_table(Table& t) {
+    if (t.env_loaded) return;
+    t.env_loaded = true;
+    const char* p = std::getenv("SGEMM_AMD_TUNE_TABLE");
+    if (p && *p) load_locked(t, p);
+}
+
+int heuristic(int form, int grouped, int m, int n) {
+    if (form == kFormF32) {
+        if (grouped) {
+            if (m >= 96 && n >= 96) return 0;
+            return m >= 96 ? 2 : 1;
+        }
+        const long long t128 = (long long)div_up(m, 128) * div_up(n, 128);
+        if (t128 >= 512) return 0;
+        const long long t64 = (long long)div_up(m, 128) * div_up(n, 64);
+        return t64 >= 512 ? 2 : 1;
+    }
+    if (grouped) return (m >= 96 && n >= 96) ? 0 : 1;
+    const long long t128 = (long long)div_up(m, 128) * div_up(n, 128);
+    return t128 >= 256 ? 0 : 1;
+}
+
+int lookup(int form, int grouped, int m, int n, int k) {
+    Table& t = table();
+    std::lock_guard<std::mutex> lock(t.mu);
+    ensure_env_table(t);
+    auto it = t.entries.find(key(form, grouped, size_bucket(m), size_bucket(n), size_bucket(k)));
+    if (it != t.entries.end()) {
+        const int c = it->second.config;
+        if (c >= 0 && c < config_count(form) && (!grouped || grouped_capable(form, c))) return c;
+    }
+    return heuristic(form, grouped, m, n);
+}
+
+int load_locked(Table& t, const char* path) {
+    FILE* f = std::fopen(path, "r");
+    if (!f) return SGEMM_STATUS_IO_ERROR;
+    char line[512];
+    std::map<int, Entry> fresh;
+    bool ok = true;
+    while (std::fgets(line, sizeof(line), f)) {
+        if (!std::strstr(line, "\"form\"")) continue;
+        int form, grouped, mb, nb, kb, cfg;
+        float tf = 0.f;
+        if (std::sscanf(line,
+                        " {\"form\": %d, \"grouped\": %d, \"mb\": %d, \"nb\": %d, \"kb\": %d, "
+                        "\"config\": %d, \"tflops\": %f",
+                        &form, &grouped, &mb, &nb, &kb, &cfg, &tf) != 7 ||
+            form < 0 || form > 1 || grouped < 0 || grouped > 1 || mb < 0 || mb > 15 || nb < 0 ||
+            nb > 15 || kb < 0 || kb > 15) {
+            ok = false;
+            break;
+        }
+        fresh[key(form, grouped, mb, nb, kb)] = Entry{cfg, tf};
+    }
+    std::fclose(f);
+    if (!ok) return SGEMM_STATUS_IO_ERROR;
+    t.entries = fresh;
+    return SGEMM_STATUS_SUCCESS;
+}
+
+// Uniform [-1, 1) from a hash of the index (tuning inputs only).
+__global__ void k_fill_tune(float* x, size_t n, unsigned seed) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        unsigned h = (unsigned)i * 2654435761u ^ seed;
+        h ^= h >> 15;
+        h *= 2246822519u;
+        h ^= h >> 13;
+        x[i] = (float)(h >> 8) * (2.0f / 16777216.0f) - 1.0f;
+    }
+}
+
+}  // namespace
+
+int size_bucket(int x) {
+    if (x <= 1) return 0;
+    int b = 0;
+    unsigned v = (unsigned)(x - 1);
+    while (v) {
+        ++b;
+        v >>= 1;
+    }
+    return b > 15 ? 15 : b;
+}
+
+int choose_config(int form, int m, int n, int k) { return lookup(form, 0, m, n, k); }
+int choose_grouped_config(int form, int m, int n, int k) { return lookup(form, 1, m, n, k); }
+
+}  // namespace sgemm_amd
+
+using namespace sgemm_amd;
+
+extern "C" {
+
+int sgemm_autotune_lookup(int form, int grouped, int m, int n, int k, int* config_id,
+                          int* from_table) {
+    if (form != kFormF32 && form != kFormBf16x3) return SGEMM_STATUS_NOT_SUPPORTED;
+    if (m < 1 || n < 1 || k < 0 || config_id == nullptr) return SGEMM_STATUS_INVALID_VALUE;
+    grouped = grouped ? 1 : 0;
+    *config_id = lookup(form, grouped, m, n, k);
+    if (from_table) {
+        Table& t = table();
+        std::lock_guard<std::mutex> lock(t.mu);
+        *from_table = t.entries.count(key(form, grouped, size_bucket(m), size_bucket(n),
+                                          size_bucket(k))) ? 1 : 0;
+    }
+    return SGEMM_STATUS_SUCCESS;
+}
+
+int sgemm_autotune_set(int form, int grouped, int m, int n, int k, int config_id) {
+    if (form != kFormF32 && form != kFormBf16x3) return SGEMM_STATUS_NOT_SUPPORTED;
+    if (m < 1 || n < 1 || k < 0) return SGEMM_STATUS_INVALID_VALUE;
+    if (config_id < 0 || config_id >= config_count(form) ||
+        (grouped && !grouped_capable(form, config_id)))
+        return SGEMM_STATUS_NOT_SUPPORTED;
+    Table& t = table();
+    std::lock_guard<std::mutex> lock(t.mu);
+    ensure_env_table(t);
+    t.entries[key(form, grouped ? 1 : 0, size_bucket(m), size_bucket(n), size_bucket(k))] =
+        Entry{config_id, 0.f};
+    return SGEMM_STATUS_SUCCESS;
+}
+
+int sgemm_autotune_clear(void) {
+    Table& t = table();
+    std::lock_guard<std::mutex> lock(t.mu);
+    t.entries.clear();
+    t.env_loaded = true;  // an explicit clear also drops the environment table
+    return SGEMM_STATUS_SUCCESS;
+}
+
+int sgemm_autotune_size(void) {
+    Table& t = table();
+    std::lock_guard<std::mutex> lock(t.mu);
+    ensure_env_table(t);
+    return (int)t.entries.size();
+}
+
+int sgemm_autotune_save(const char* path) {
+    if (path == nullptr) return SGEMM_STATUS_INVALID_VALUE;
+    Table& t = table();
+    std::lock_guard<std::mutex> lock(t.mu);
+    ensure_env_table(t);
+    FILE* f = std::fopen(path, "w");
+    if (!f) return SGEMM_STATUS_IO_ERROR;
+    std::fprintf(f, "{\"format\": \"sgemm_amd autotune table v1\",\n \"entries\": [\n");
+    size_t i = 0;
+    for (const auto& kv : t.entries) {
+        int form, grouped, mb, nb, kb;
+        unkey(kv.first, form, grouped, mb, nb, kb);
+        std::fprintf(f,
+                     "  {\"form\": %d, \"grouped\": %d, \"mb\": %d, \"nb\": %d, \"kb\": %d, "
+                     "\"config\": %d, \"tflops\": %.3f}%s\n",
+                     form, grouped, mb, nb, kb, kv.second.config, kv.second.tflops,
+                     ++i < t.entries.size() ? "," : "");
+    }
+    std::fprintf(f, " ]\n}\n");
+    return std::fclose(f) == 0 ? SGEMM_STATUS_SUCCESS : SGEMM_STATUS_IO_ERROR;
+}
+
+int sgemm_autotune_load(const char* path) {
+    if (path == nullptr) return SGEMM_STATUS_INVALID_VALUE;
+    Table& t = table();
+    std::lock_guard<std::mutex> lock(t.mu);
+    t.env_loaded = true;
+    return load_locked(t, path);
+}
+
+int sgemm_autotune_bucket(int form, int grouped, int m, int n, int k, int iters,
+                          sgemm_stream_t stream, int* best_config, float* best_tflops) {
+    if (form != kFormF32 && form != kFormBf16x3) return SGEMM_STATUS_NOT_SUPPORTED;
+    if (m < 1 || n < 1 || k < 1 || iters < 1) return SGEMM_STATUS_INVALID_VALUE;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t na = (size_t)m * k, nb = (size_t)k * n, nc = (size_t)m * n;
+    float *A = nullptr, *B = nullptr, *C = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int status = SGEMM_STATUS_SUCCESS;
+    if (hipMalloc(&A, na * 4) != hipSuccess || hipMalloc(&B, nb * 4) != hipSuccess ||
+        hipMalloc(&C, nc * 4) != hipSuccess) {
+        status = SGEMM_STATUS_ALLOC_FAILED;
+    } else if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        status = SGEMM_STATUS_HIP_ERROR;
+    }
+    int best = -1;
+    float best_tf = 0.f;
+    if (status == SGEMM_STATUS_SUCCESS) {
+        hipLaunchKernelGGL(k_fill_tune, dim3(1024), dim3(256), 0, s, A, na, 1u);
+        hipLaunchKernelGGL(k_fill_tune, dim3(1024), dim3(256), 0, s, B, nb, 2u);
+        const float one = 1.f, zero = 0.f;
+        const double flops = 2.0 * m * n * (double)k;
+        for (int c = 0; c < config_count(form); ++c) {
+            if (grouped && !grouped_capable(form, c)) continue;
+            // warm-up, then the timed repetitions
+            int st = sgemm_run_config(form, c, m, n, k, &one, A, m, B, k, &zero, C, m, s);
+            if (st != SGEMM_STATUS_SUCCESS) continue;
+            hipEventRecord(e0, s);
+            for (int r = 0; r < iters; ++r)
+                sgemm_run_config(form, c, m, n, k, &one, A, m, B, k, &zero, C, m, s);
+            hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                status = SGEMM_STATUS_HIP_ERROR;
+                break;
+            }
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            const float tf = (float)(flops * iters / (ms * 1e-3) / 1e12);
+            if (best < 0 || tf > best_tf) {
+                best = c;
+                best_tf = tf;
+            }
+        }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(A);
+    hipFree(B);
+    hipFree(C);
+    if (status == SGEMM_STATUS_SUCCESS && best < 0) status = SGEMM_STATUS_HIP_ERROR;
+    if (status == SGEMM_STATUS_SUCCESS) {
+        Table& t = table();
+        std::lock_guard<std::mutex> lock(t.mu);
+        ensure_env_table(t);
+        t.entries[key(form, grouped ? 1 : 0, size_bucket(m), size_bucket(n), size_bucket(k))] =
+            Entry{best, best_tf};
+        if (best_config) *best_config = best;
+        if (best_tflops) *best_tflops = best_tf;
+    }
+    return status;
+}
+
+}  // extern "C"

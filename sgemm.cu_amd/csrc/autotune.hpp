@@ -1,10 +1,18 @@
-// autotune.hpp -- config selection per (form, m, n, k).
+// autotune.hpp -- config selection per (form, m, n, k) size bucket.
 #pragma once
 
 namespace sgemm_amd {
 
-// Config id for a single problem of this shape: the autotune table's entry
-// for the shape's size bucket, else the built-in heuristic.
+// Size bucket of one dimension: ceil(log2(x)) capped to [0, 15]
+// (1 -> 0, 2 -> 1, 3..4 -> 2, ..., 8193..16384 -> 14, larger -> 15).
+int size_bucket(int x);
+
+// Config id for a single-problem launch of this shape: the table entry of the
+// shape's bucket if there is one, else the built-in heuristic.
 int choose_config(int form, int m, int n, int k);
+
+// Same for a problem inside a grouped launch (restricted to the configs the
+// grouped kernel carries).
+int choose_grouped_config(int form, int m, int n, int k);
 
 }  // namespace sgemm_amd

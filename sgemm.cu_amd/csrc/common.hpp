@@ -26,7 +26,11 @@ struct Problem {
     int tiles_m, tiles_n;  // tile grid of this problem for the chosen config
     int tile_begin;        // first global tile index (grouped schedules)
     int cfg;               // config id the tiles were planned with
+    int flags;             // kProblemVec4: every 16-byte vector access is aligned
+    int reserved;
 };
+
+constexpr int kProblemVec4 = 1;
 
 static_assert(sizeof(Problem) % 8 == 0, "descriptor packing");
 

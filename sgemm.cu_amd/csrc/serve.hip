@@ -1,0 +1,540 @@
+// serve.hip -- continuous-batching GEMM server (no reference counterpart; the
+// reference has no serving layer).
+//
+// Requests arrive through sgemm_server_submit and queue in arrival order.  A
+// scheduler thread admits them against a fixed HBM budget and packs them into
+// grouped launches (grouped.hip), one launch per "slot"; every slot has its
+// own stream, so up to num_slots launches overlap on the device.  A request
+// that arrives while others are on the GPU is admitted as soon as a slot and
+// enough of the budget are free -- it does not wait for the running batches.
+//
+// Admission is strict FIFO: the head request is admitted when its footprint
+// fits the free budget; later requests never overtake it, so nothing starves.
+// Footprint = 4*(m*k + k*n + m*n) bytes (the request's operands and result).
+//   * SGEMM_OPERANDS_HOST: the server owns the device copies.  It carves them
+//     out of one pool allocated at create time (first-fit, coalescing free
+//     list), copies A, B (and C when beta != 0) in on the slot's stream, and C
+//     back after the launch.
+//   * SGEMM_OPERANDS_DEVICE: the operands already live on the device; the
+//     footprint is reserved against the budget while the request is in flight.
+//
+// Every request's config comes from the autotune table for its own shape, so
+// its bits do not depend on which other requests share its launch.
+//
+// The executor is replaceable: SGEMM_EXECUTOR_MOCK completes a batch after
+// flops / mock_tflops seconds without touching a GPU, so admission, batching
+// and the statistics are testable on any host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/sgemm.h"
+#include "grouped.hpp"
+
+namespace sgemm_amd {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double seconds_since(Clock::time_point t0) {
+    return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// First-fit allocator over [0, capacity) with coalescing; offsets only, so the
+// mock executor can use it without device memory.
+class RangeAllocator {
+  public:
+    explicit RangeAllocator(size_t capacity) { free_[0] = capacity; }
+    bool alloc(size_t bytes, size_t* off) {
+        bytes = align_up(std::max<size_t>(bytes, 1), 256);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                *off = it->first;
+                const size_t rest = it->second - bytes;
+                const size_t next = it->first + bytes;
+                free_.erase(it);
+                if (rest) free_[next] = rest;
+                used_[*off] = bytes;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off) {
+        auto u = used_.find(off);
+        if (u == used_.end()) return;
+        size_t start = off, len = u->second;
+        used_.erase(u);
+        auto next = free_.lower_bound(start);
+        if (next != free_.end() && start + len == next->first) {
+            len += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == start) {
+                prev->second += len;
+                return;
+            }
+        }
+        free_[start] = len;
+    }
+
+  private:
+    std::map<size_t, size_t> free_;  // offset -> length
+    std::map<size_t, size_t> used_;
+};
+
+struct Req {
+    uint64_t id = 0;
+    sgemm_request r{};
+    size_t footprint = 0;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    int status = SGEMM_STATUS_NOT_READY;
+    bool pooled = false;
+    size_t off[3] = {0, 0, 0};  // pool offsets of A, B, C (host operands)
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    GroupedPlan* plan = nullptr;
+    std::vector<Req*> batch;
+    bool busy = false;
+    double mock_done_at = 0;
+};
+
+}  // namespace
+
+class Server {
+  public:
+    explicit Server(const sgemm_server_config& c) : cfg_(c), alloc_(c.hbm_budget_bytes) {}
+
+    int start() {
+        t0_ = Clock::now();
+        const bool gpu = cfg_.executor == SGEMM_EXECUTOR_GPU;
+        if (gpu) {
+            if (hipMalloc(&pool_, std::max<size_t>(cfg_.hbm_budget_bytes, 256)) != hipSuccess) {
+                (void)hipGetLastError();
+                pool_ = nullptr;
+                return SGEMM_STATUS_ALLOC_FAILED;
+            }
+        }
+        slots_.resize(cfg_.num_slots);
+        for (Slot& s : slots_) {
+            if (!gpu) continue;
+            if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess)
+                return SGEMM_STATUS_HIP_ERROR;
+            int st = GroupedPlan::create(&s.plan, cfg_.max_batch, cfg_.form);
+            if (st != SGEMM_STATUS_SUCCESS) return st;
+        }
+        worker_ = std::thread([this] { loop(); });
+        return SGEMM_STATUS_SUCCESS;
+    }
+
+    ~Server() {
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (worker_.joinable()) worker_.join();
+        for (Slot& s : slots_) {
+            if (s.stream) hipStreamSynchronize(s.stream);
+            if (s.plan) {
+                s.plan->release();
+                delete s.plan;
+            }
+            if (s.done) hipEventDestroy(s.done);
+            if (s.stream) hipStreamDestroy(s.stream);
+        }
+        if (pool_) hipFree(pool_);
+    }
+
+    int submit(const sgemm_request& r, uint64_t* id) {
+        sgemm_problem q{r.m, r.n, r.k, r.alpha, r.beta, r.A, r.lda, r.B, r.ldb, r.C, r.ldc};
+        int st = validate_problem(q);
+        if (st != SGEMM_STATUS_SUCCESS) return st;
+        if (r.operands != SGEMM_OPERANDS_DEVICE && r.operands != SGEMM_OPERANDS_HOST)
+            return SGEMM_STATUS_INVALID_VALUE;
+        auto q_ = std::make_unique<Req>();
+        q_->r = r;
+        q_->footprint = footprint(r);
+        if (q_->footprint > cfg_.hbm_budget_bytes) {
+            std::lock_guard<std::mutex> lock(mu_);
+            ++rejected_;
+            return SGEMM_STATUS_BUDGET_EXCEEDED;
+        }
+        std::lock_guard<std::mutex> lock(mu_);
+        q_->id = ++next_id_;
+        q_->t_submit = seconds_since(t0_);
+        if (id) *id = q_->id;
+        pending_.push_back(q_.get());
+        all_[q_->id] = std::move(q_);
+        ++submitted_;
+        cv_.notify_all();
+        return SGEMM_STATUS_SUCCESS;
+    }
+
+    int poll(uint64_t id) {
+        std::lock_guard<std::mutex> lock(mu_);
+        auto it = all_.find(id);
+        return it == all_.end() ? SGEMM_STATUS_NOT_FOUND : it->second->status;
+    }
+
+    int wait(uint64_t id, double timeout_s) {
+        std::unique_lock<std::mutex> lock(mu_);
+        auto it = all_.find(id);
+        if (it == all_.end()) return SGEMM_STATUS_NOT_FOUND;
+        Req* q = it->second.get();
+        const auto deadline = Clock::now() + std::chrono::duration<double>(timeout_s);
+        while (q->status == SGEMM_STATUS_NOT_READY) {
+            if (done_cv_.wait_until(lock, deadline) == std::cv_status::timeout &&
+                q->status == SGEMM_STATUS_NOT_READY)
+                return SGEMM_STATUS_NOT_READY;
+        }
+        return q->status;
+    }
+
+    int drain(double timeout_s) {
+        std::unique_lock<std::mutex> lock(mu_);
+        const auto deadline = Clock::now() + std::chrono::duration<double>(timeout_s);
+        while (completed_ + failed_ < submitted_) {
+            if (done_cv_.wait_until(lock, deadline) == std::cv_status::timeout &&
+                completed_ + failed_ < submitted_)
+                return SGEMM_STATUS_NOT_READY;
+        }
+        return failed_ ? SGEMM_STATUS_HIP_ERROR : SGEMM_STATUS_SUCCESS;
+    }
+
+    int latency(uint64_t id, double* total_ms, double* queue_ms) {
+        std::lock_guard<std::mutex> lock(mu_);
+        auto it = all_.find(id);
+        if (it == all_.end()) return SGEMM_STATUS_NOT_FOUND;
+        const Req* q = it->second.get();
+        if (q->status != SGEMM_STATUS_SUCCESS) return q->status;
+        if (total_ms) *total_ms = (q->t_done - q->t_submit) * 1e3;
+        if (queue_ms) *queue_ms = (q->t_admit - q->t_submit) * 1e3;
+        return SGEMM_STATUS_SUCCESS;
+    }
+
+    void stats(sgemm_server_stats* out) {
+        std::lock_guard<std::mutex> lock(mu_);
+        std::memset(out, 0, sizeof(*out));
+        out->submitted = submitted_;
+        out->completed = completed_;
+        out->rejected = rejected_;
+        out->failed = failed_;
+        out->batches = batches_;
+        out->mean_batch = batches_ ? (double)batched_requests_ / (double)batches_ : 0.0;
+        out->peak_bytes = peak_bytes_;
+        out->budget_bytes = cfg_.hbm_budget_bytes;
+        std::vector<double> lat;
+        double qsum = 0, flops = 0, first = 1e300, last = 0;
+        for (const auto& kv : all_) {
+            const Req* q = kv.second.get();
+            if (q->status != SGEMM_STATUS_SUCCESS || q->t_submit < stats_epoch_) continue;
+            lat.push_back((q->t_done - q->t_submit) * 1e3);
+            qsum += (q->t_admit - q->t_submit) * 1e3;
+            flops += 2.0 * q->r.m * q->r.n * (double)q->r.k;
+            first = std::min(first, q->t_submit);
+            last = std::max(last, q->t_done);
+        }
+        out->measured = (long long)lat.size();
+        if (lat.empty()) return;
+        std::sort(lat.begin(), lat.end());
+        auto pct = [&](double p) {
+            // nearest-rank percentile
+            size_t r = (size_t)std::ceil(p / 100.0 * (double)lat.size());
+            return lat[std::min(lat.size(), std::max<size_t>(r, 1)) - 1];
+        };
+        out->p50_ms = pct(50);
+        out->p99_ms = pct(99);
+        out->max_ms = lat.back();
+        double s = 0;
+        for (double v : lat) s += v;
+        out->mean_ms = s / (double)lat.size();
+        out->mean_queue_ms = qsum / (double)lat.size();
+        out->window_s = last - first;
+        out->tflops = last > first ? flops / (last - first) / 1e12 : 0.0;
+    }
+
+    void reset_stats() {
+        std::lock_guard<std::mutex> lock(mu_);
+        stats_epoch_ = seconds_since(t0_);
+        peak_bytes_ = used_bytes_;
+        batches_ = 0;
+        batched_requests_ = 0;
+    }
+
+  private:
+    static size_t footprint(const sgemm_request& r) {
+        return 4 * ((size_t)r.m * r.k + (size_t)r.k * r.n + (size_t)r.m * r.n);
+    }
+
+    // Scheduler thread.
+    void loop() {
+        std::unique_lock<std::mutex> lock(mu_);
+        while (!stop_) {
+            retire();
+            admit();
+            bool any_busy = false;
+            for (const Slot& s : slots_) any_busy |= s.busy;
+            if (any_busy)
+                cv_.wait_for(lock, std::chrono::microseconds(20));
+            else
+                cv_.wait_for(lock, std::chrono::milliseconds(50));
+        }
+        // Let in-flight work finish before the slots are torn down.
+        for (Slot& s : slots_)
+            if (s.busy && s.done) {
+                lock.unlock();
+                hipEventSynchronize(s.done);
+                lock.lock();
+            }
+    }
+
+    bool slot_done(Slot& s) {
+        if (cfg_.executor == SGEMM_EXECUTOR_MOCK) return seconds_since(t0_) >= s.mock_done_at;
+        hipError_t e = hipEventQuery(s.done);
+        if (e == hipErrorNotReady) return false;
+        if (e != hipSuccess) {
+            for (Req* q : s.batch) q->status = SGEMM_STATUS_HIP_ERROR;
+        }
+        return true;
+    }
+
+    void retire() {
+        bool any = false;
+        for (Slot& s : slots_) {
+            if (!s.busy || !slot_done(s)) continue;
+            const double now = seconds_since(t0_);
+            for (Req* q : s.batch) {
+                q->t_done = now;
+                if (q->status == SGEMM_STATUS_NOT_READY) {
+                    q->status = SGEMM_STATUS_SUCCESS;
+                    ++completed_;
+                } else {
+                    ++failed_;
+                }
+                release(q);
+            }
+            s.batch.clear();
+            s.busy = false;
+            any = true;
+        }
+        if (any) done_cv_.notify_all();
+    }
+
+    void release(Req* q) {
+        if (q->pooled)
+            for (size_t o : q->off) alloc_.release(o);
+        q->pooled = false;
+        used_bytes_ -= q->footprint;
+    }
+
+    // Reserve the head request's footprint (and pool space for host operands).
+    bool reserve(Req* q) {
+        if (used_bytes_ + q->footprint > cfg_.hbm_budget_bytes) return false;
+        if (q->r.operands == SGEMM_OPERANDS_HOST) {
+            const size_t sz[3] = {4 * (size_t)q->r.m * q->r.k, 4 * (size_t)q->r.k * q->r.n,
+                                  4 * (size_t)q->r.m * q->r.n};
+            int got = 0;
+            for (; got < 3; ++got)
+                if (!alloc_.alloc(sz[got], &q->off[got])) break;
+            if (got < 3) {
+                for (int i = 0; i < got; ++i) alloc_.release(q->off[i]);
+                return false;
+            }
+            q->pooled = true;
+        }
+        used_bytes_ += q->footprint;
+        peak_bytes_ = std::max(peak_bytes_, used_bytes_);
+        return true;
+    }
+
+    void admit() {
+        for (Slot& s : slots_) {
+            if (s.busy) continue;
+            if (pending_.empty()) return;
+            std::vector<Req*> batch;
+            while (!pending_.empty() && (int)batch.size() < cfg_.max_batch) {
+                Req* q = pending_.front();
+                if (!reserve(q)) break;  // strict FIFO: the head waits for memory
+                q->t_admit = seconds_since(t0_);
+                pending_.pop_front();
+                batch.push_back(q);
+            }
+            if (batch.empty()) return;
+            s.batch = batch;
+            s.busy = true;
+            ++batches_;
+            batched_requests_ += (long long)batch.size();
+            if (launch(s) != SGEMM_STATUS_SUCCESS) {
+                for (Req* q : s.batch) q->status = SGEMM_STATUS_HIP_ERROR;
+                s.mock_done_at = 0;  // retire at once
+                if (cfg_.executor == SGEMM_EXECUTOR_GPU) hipEventRecord(s.done, s.stream);
+            }
+        }
+    }
+
+    int launch(Slot& s) {
+        double flops = 0;
+        for (Req* q : s.batch) flops += 2.0 * q->r.m * q->r.n * (double)q->r.k;
+        if (cfg_.executor == SGEMM_EXECUTOR_MOCK) {
+            s.mock_done_at = seconds_since(t0_) + cfg_.mock_launch_us * 1e-6 +
+                             flops / (cfg_.mock_tflops * 1e12);
+            return SGEMM_STATUS_SUCCESS;
+        }
+        std::vector<sgemm_problem> probs;
+        probs.reserve(s.batch.size());
+        for (Req* q : s.batch) {
+            const sgemm_request& r = q->r;
+            sgemm_problem p{r.m, r.n, r.k, r.alpha, r.beta, r.A, r.lda, r.B, r.ldb, r.C, r.ldc};
+            if (r.operands == SGEMM_OPERANDS_HOST) {
+                char* base = static_cast<char*>(pool_);
+                float* dA = reinterpret_cast<float*>(base + q->off[0]);
+                float* dB = reinterpret_cast<float*>(base + q->off[1]);
+                float* dC = reinterpret_cast<float*>(base + q->off[2]);
+                // pack host operands (column-major, ld) into dense device copies
+                if (hipMemcpy2DAsync(dA, 4 * (size_t)r.m, r.A, 4 * (size_t)r.lda,
+                                     4 * (size_t)r.m, r.k, hipMemcpyHostToDevice,
+                                     s.stream) != hipSuccess ||
+                    hipMemcpy2DAsync(dB, 4 * (size_t)r.k, r.B, 4 * (size_t)r.ldb,
+                                     4 * (size_t)r.k, r.n, hipMemcpyHostToDevice,
+                                     s.stream) != hipSuccess)
+                    return SGEMM_STATUS_HIP_ERROR;
+                if (r.beta != 0.f &&
+                    hipMemcpy2DAsync(dC, 4 * (size_t)r.m, r.C, 4 * (size_t)r.ldc,
+                                     4 * (size_t)r.m, r.n, hipMemcpyHostToDevice,
+                                     s.stream) != hipSuccess)
+                    return SGEMM_STATUS_HIP_ERROR;
+                p.A = dA;
+                p.lda = std::max(r.m, 1);
+                p.B = dB;
+                p.ldb = std::max(r.k, 1);
+                p.C = dC;
+                p.ldc = std::max(r.m, 1);
+            }
+            probs.push_back(p);
+        }
+        int st = s.plan->set((int)probs.size(), probs.data(), s.stream, nullptr);
+        if (st == SGEMM_STATUS_SUCCESS) st = s.plan->run(s.stream);
+        if (st != SGEMM_STATUS_SUCCESS) return st;
+        for (size_t i = 0; i < s.batch.size(); ++i) {
+            const Req* q = s.batch[i];
+            const sgemm_request& r = q->r;
+            if (r.operands != SGEMM_OPERANDS_HOST) continue;
+            if (hipMemcpy2DAsync(r.C, 4 * (size_t)r.ldc, probs[i].C, 4 * (size_t)r.m,
+                                 4 * (size_t)r.m, r.n, hipMemcpyDeviceToHost,
+                                 s.stream) != hipSuccess)
+                return SGEMM_STATUS_HIP_ERROR;
+        }
+        return hipEventRecord(s.done, s.stream) == hipSuccess ? SGEMM_STATUS_SUCCESS
+                                                              : SGEMM_STATUS_HIP_ERROR;
+    }
+
+    sgemm_server_config cfg_;
+    RangeAllocator alloc_;
+    void* pool_ = nullptr;
+    std::vector<Slot> slots_;
+    std::deque<Req*> pending_;
+    std::unordered_map<uint64_t, std::unique_ptr<Req>> all_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    std::thread worker_;
+    bool stop_ = false;
+    Clock::time_point t0_;
+    uint64_t next_id_ = 0;
+    long long submitted_ = 0, completed_ = 0, rejected_ = 0, failed_ = 0;
+    long long batches_ = 0, batched_requests_ = 0;
+    size_t used_bytes_ = 0, peak_bytes_ = 0;
+    double stats_epoch_ = 0;
+};
+
+}  // namespace sgemm_amd
+
+using sgemm_amd::Server;
+
+extern "C" {
+
+int sgemm_server_create(sgemm_server* srv, const sgemm_server_config* cfg) {
+    if (srv == nullptr || cfg == nullptr) return SGEMM_STATUS_INVALID_VALUE;
+    *srv = nullptr;
+    if (cfg->hbm_budget_bytes == 0 || cfg->max_batch < 1 || cfg->num_slots < 1 ||
+        cfg->num_slots > 16)
+        return SGEMM_STATUS_INVALID_VALUE;
+    if (cfg->form != SGEMM_FORM_F32 && cfg->form != SGEMM_FORM_BF16X3)
+        return SGEMM_STATUS_NOT_SUPPORTED;
+    if (cfg->executor != SGEMM_EXECUTOR_GPU && cfg->executor != SGEMM_EXECUTOR_MOCK)
+        return SGEMM_STATUS_INVALID_VALUE;
+    if (cfg->executor == SGEMM_EXECUTOR_MOCK && !(cfg->mock_tflops > 0))
+        return SGEMM_STATUS_INVALID_VALUE;
+    Server* s = new Server(*cfg);
+    int st = s->start();
+    if (st != SGEMM_STATUS_SUCCESS) {
+        delete s;
+        return st;
+    }
+    *srv = reinterpret_cast<sgemm_server>(s);
+    return SGEMM_STATUS_SUCCESS;
+}
+
+int sgemm_server_submit(sgemm_server srv, const sgemm_request* req, uint64_t* id) {
+    if (srv == nullptr || req == nullptr) return SGEMM_STATUS_INVALID_VALUE;
+    return reinterpret_cast<Server*>(srv)->submit(*req, id);
+}
+
+int sgemm_server_poll(sgemm_server srv, uint64_t id) {
+    if (srv == nullptr) return SGEMM_STATUS_INVALID_VALUE;
+    return reinterpret_cast<Server*>(srv)->poll(id);
+}
+
+int sgemm_server_wait(sgemm_server srv, uint64_t id, double timeout_s) {
+    if (srv == nullptr) return SGEMM_STATUS_INVALID_VALUE;
+    return reinterpret_cast<Server*>(srv)->wait(id, timeout_s);
+}
+
+int sgemm_server_drain(sgemm_server srv, double timeout_s) {
+    if (srv == nullptr) return SGEMM_STATUS_INVALID_VALUE;
+    return reinterpret_cast<Server*>(srv)->drain(timeout_s);
+}
+
+int sgemm_server_latency(sgemm_server srv, uint64_t id, double* total_ms, double* queue_ms) {
+    if (srv == nullptr) return SGEMM_STATUS_INVALID_VALUE;
+    return reinterpret_cast<Server*>(srv)->latency(id, total_ms, queue_ms);
+}
+
+int sgemm_server_get_stats(sgemm_server srv, sgemm_server_stats* out) {
+    if (srv == nullptr || out == nullptr) return SGEMM_STATUS_INVALID_VALUE;
+    reinterpret_cast<Server*>(srv)->stats(out);
+    return SGEMM_STATUS_SUCCESS;
+}
+
+int sgemm_server_reset_stats(sgemm_server srv) {
+    if (srv == nullptr) return SGEMM_STATUS_INVALID_VALUE;
+    reinterpret_cast<Server*>(srv)->reset_stats();
+    return SGEMM_STATUS_SUCCESS;
+}
+
+int sgemm_server_destroy(sgemm_server srv) {
+    if (srv == nullptr) return SGEMM_STATUS_INVALID_VALUE;
+    delete reinterpret_cast<Server*>(srv);
+    return SGEMM_STATUS_SUCCESS;
+}
+
+}  // extern "C"

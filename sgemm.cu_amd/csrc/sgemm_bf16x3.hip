@@ -1,14 +1,48 @@
-// sgemm_bf16x3.hip -- split-precision BF16x3 SGEMM on the BF16 matrix cores.
-#include "kernels.hpp"
+// sgemm_bf16x3.hip -- single-problem launches of the BF16x3 tile (tile_x3.hpp).
+#include "tile_f32.hpp"
+#include "tile_x3.hpp"
 
 namespace sgemm_amd {
 
+constexpr int kX3GroupN = 8;
+
+template <class T, bool VEC>
+__global__ __launch_bounds__(T::NT) void k_sgemm_bf16x3(Problem P) {
+    __shared__ __attribute__((aligned(16))) char smem[T::LDS_BYTES];
+    const int id = xcd_remap(blockIdx.x, gridDim.x);
+    int tm, tn;
+    tile_coords(id, P.tiles_m, P.tiles_n, kX3GroupN, tm, tn);
+    X3TileRunner<T, VEC>::run(P, tm, tn, smem);
+}
+
 static const ConfigInfo kBf16x3Configs[] = {
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2"},
+    {64, 64, 64, 2, 2, "bf16x3_64x64x64_w2x2"},
 };
 
-int bf16x3_config_count() { return 0; }
+int bf16x3_config_count() { return (int)(sizeof(kBf16x3Configs) / sizeof(kBf16x3Configs[0])); }
 const ConfigInfo& bf16x3_config(int id) { return kBf16x3Configs[id]; }
-hipError_t launch_bf16x3(int, const Problem&, hipStream_t) { return hipErrorNotSupported; }
+
+template <class T>
+static hipError_t launch_x3(const Problem& p, hipStream_t s) {
+    Problem q = p;
+    q.tiles_m = div_up(p.m, T::BM);
+    q.tiles_n = div_up(p.n, T::BN);
+    const long long tiles = (long long)q.tiles_m * q.tiles_n;
+    if (tiles <= 0) return hipSuccess;
+    if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
+    (void)hipGetLastError();
+    if (vec4_ok(p))
+        hipLaunchKernelGGL((k_sgemm_bf16x3<T, true>), dim3((unsigned)tiles), dim3(T::NT), 0, s, q);
+    else
+        hipLaunchKernelGGL((k_sgemm_bf16x3<T, false>), dim3((unsigned)tiles), dim3(T::NT), 0, s, q);
+    return hipGetLastError();
+}
+
+hipError_t launch_bf16x3(int id, const Problem& p, hipStream_t s) {
+    hipError_t e = hipErrorInvalidValue;
+    dispatch_x3(id, [&](auto tag) { e = launch_x3<typename decltype(tag)::type>(p, s); });
+    return e;
+}
 
 }  // namespace sgemm_amd

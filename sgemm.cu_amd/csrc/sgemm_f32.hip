@@ -1,199 +1,8 @@
-// sgemm_f32.hip -- plain FP32 SGEMM for gfx950 on the FP32 matrix cores.
-//
-// C := alpha*A*B + beta*C, column-major.  Replaces the reference's FFMA kernels
-// sgemm_128x256x8 / sgemm_128x128x8 (src/kernels/128x256x8.cuh:4-387,
-// src/kernels/128x128x8.cuh:4-386) with a wave64 design:
-//
-//  * The inner product runs on v_mfma_f32_16x16x4_f32: FP32 inputs, exact FP32
-//    products, FP32 accumulation; each instruction is a rank-4 update of a
-//    16x16 block held in 4 accumulator registers per lane.  A wave owns a
-//    (BM/WM) x (BN/WN) register-blocked tile of such blocks.
-//  * Global -> LDS: coalesced 16-byte loads into registers for k-tile t+1 while
-//    the matrix cores work on tile t out of the other LDS stage (double
-//    buffer, one barrier per k-tile).  A is kept m-contiguous in LDS (as in
-//    HBM), B k-contiguous (as in HBM); row paddings make the fragment reads
-//    bank-conflict free.
-//  * Every edge is guarded: rows/cols/k outside the problem load as zero and
-//    are never stored; ragged K is the last (zero-padded) k-tile.  Unlike the
-//    reference (which uses n for ldb/ldc, 128x128x8.cuh:263,368), every leading
-//    dimension is honoured.
-//  * Fused epilogue: C = alpha*acc + beta*C; beta == 0 never reads C
-//    (128x128x8.cuh:364-380 semantics).
-//  * Barriers are executed by every wave of the workgroup (no barrier inside a
-//    data-dependent branch, unlike 128x128x8.cuh:344-358).
-#include "kernels.hpp"
+// sgemm_f32.hip -- single-problem launches of the plain FP32 tile
+// (tile_f32.hpp), the naive reference-order kernel and the beta-scale kernel.
+#include "tile_f32.hpp"
 
 namespace sgemm_amd {
-
-template <int BM_, int BN_, int BK_, int WM_, int WN_>
-struct F32Tile {
-    static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
-    static constexpr int NT = 64 * WM * WN;
-    static constexpr int TM = BM / WM, TN = BN / WN;  // wave tile
-    static constexpr int MB = TM / 16, NB = TN / 16;  // 16x16 MFMA blocks per wave
-    static constexpr int SA = BM + 4;                 // As[k][m] row stride (floats)
-    static constexpr int SB = BK + 4;                 // Bs[n][k] row stride (floats)
-    static constexpr int STAGE = BK * SA + BN * SB;   // floats per LDS stage
-    static constexpr int LDS_FLOATS = 2 * STAGE;
-    static constexpr int A_VEC = BM * BK / 4 / NT;  // float4 of A per thread per k-tile
-    static constexpr int B_VEC = BN * BK / 4 / NT;
-    static constexpr int A_KSTEP = NT / (BM / 4);   // k rows covered by one pass of threads
-    static constexpr int B_NSTEP = NT / (BK / 4);   // n columns covered by one pass
-    static_assert(TM % 16 == 0 && TN % 16 == 0, "wave tile must be 16-aligned");
-    static_assert(BK % 16 == 0, "BK must be a multiple of 16");
-    static_assert((BM * BK / 4) % NT == 0 && (BN * BK / 4) % NT == 0, "load split");
-    static_assert(NT % (BM / 4) == 0 && NT % (BK / 4) == 0, "load mapping");
-    static_assert(LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
-};
-
-// Rows r..r+3 of one column, zero outside [0, valid).  VEC: one 16-byte load
-// when all four are valid.
-template <bool VEC>
-__device__ inline f32x4 load4(const float* p, int valid) {
-    f32x4 v = {0.f, 0.f, 0.f, 0.f};
-    if (VEC && valid >= 4) return *reinterpret_cast<const f32x4*>(p);
-    if (valid > 0) v[0] = p[0];
-    if (valid > 1) v[1] = p[1];
-    if (valid > 2) v[2] = p[2];
-    if (valid > 3) v[3] = p[3];
-    return v;
-}
-
-template <class T, bool VEC>
-struct F32TileRunner {
-    static constexpr int BM = T::BM, BN = T::BN, BK = T::BK;
-
-    // Computes tile (tm, tn) of problem P.  `smem` holds T::LDS_FLOATS floats.
-    // All threads of the workgroup must call this together.
-    __device__ static void run(const Problem& P, int tm, int tn, float* smem) {
-        const int t = threadIdx.x;
-        const int lane = t & 63, wave = t >> 6;
-        const int wm = wave % T::WM, wn = wave / T::WM;
-        const int l16 = lane & 15, g = lane >> 4;
-        const int m0 = tm * BM, n0 = tn * BN;
-        const int M = P.m, N = P.n, K = P.k;
-
-        // ---- per-thread global load mapping (fixed across k-tiles)
-        const int a_m = 4 * (t % (BM / 4));  // row inside the tile
-        const int a_k = t / (BM / 4);        // first k row inside the tile
-        const int a_valid = min(max(M - (m0 + a_m), 0), 4);
-        const float* a_src = P.A + (size_t)(m0 + (a_valid > 0 ? a_m : 0)) + (size_t)a_k * P.lda;
-        const size_t a_kstride = (size_t)T::A_KSTEP * P.lda;
-
-        const int b_k = 4 * (t % (BK / 4));
-        const int b_n = t / (BK / 4);
-        const float* b_src[T::B_VEC];
-        bool b_colok[T::B_VEC];
-#pragma unroll
-        for (int r = 0; r < T::B_VEC; ++r) {
-            const int col = n0 + b_n + r * T::B_NSTEP;
-            b_colok[r] = col < N;
-            b_src[r] = P.B + (size_t)(b_colok[r] ? col : 0) * P.ldb + b_k;
-        }
-
-        f32x4 ra[T::A_VEC], rb[T::B_VEC];
-
-        auto gload = [&](int k0) {
-#pragma unroll
-            for (int r = 0; r < T::A_VEC; ++r) {
-                const int kk = k0 + a_k + r * T::A_KSTEP;
-                const int valid = kk < K ? a_valid : 0;
-                ra[r] = load4<VEC>(a_src + (size_t)k0 * P.lda + r * a_kstride, valid);
-            }
-#pragma unroll
-            for (int r = 0; r < T::B_VEC; ++r) {
-                const int valid = b_colok[r] ? min(max(K - (k0 + b_k), 0), 4) : 0;
-                rb[r] = load4<VEC>(b_src[r] + k0, valid);
-            }
-        };
-        auto sstore = [&](float* stage) {
-            float* As = stage;
-            float* Bs = stage + BK * T::SA;
-#pragma unroll
-            for (int r = 0; r < T::A_VEC; ++r)
-                *reinterpret_cast<f32x4*>(&As[(a_k + r * T::A_KSTEP) * T::SA + a_m]) = ra[r];
-#pragma unroll
-            for (int r = 0; r < T::B_VEC; ++r)
-                *reinterpret_cast<f32x4*>(&Bs[(b_n + r * T::B_NSTEP) * T::SB + b_k]) = rb[r];
-        };
-
-        f32x4 acc[T::MB][T::NB];
-#pragma unroll
-        for (int i = 0; i < T::MB; ++i)
-#pragma unroll
-            for (int j = 0; j < T::NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-        const int ktiles = div_up(K, BK);
-        if (ktiles > 0) {
-            gload(0);
-            sstore(smem);
-            __syncthreads();
-        }
-        for (int kt = 0; kt < ktiles; ++kt) {
-            const float* As = smem + (kt & 1) * T::STAGE;
-            const float* Bs = As + BK * T::SA;
-            const bool more = kt + 1 < ktiles;
-            if (more) gload((kt + 1) * BK);
-#pragma unroll
-            for (int kk = 0; kk < BK; kk += 16) {
-                // Lane group g supplies k = kk + 4g + s at MFMA step s, for A
-                // and B alike, so B's four steps are one 16-byte LDS read.
-                float a[T::MB][4];
-                f32x4 b[T::NB];
-#pragma unroll
-                for (int i = 0; i < T::MB; ++i)
-#pragma unroll
-                    for (int s = 0; s < 4; ++s)
-                        a[i][s] = As[(kk + 4 * g + s) * T::SA + wm * T::TM + 16 * i + l16];
-#pragma unroll
-                for (int j = 0; j < T::NB; ++j)
-                    b[j] = *reinterpret_cast<const f32x4*>(
-                        &Bs[(wn * T::TN + 16 * j + l16) * T::SB + kk + 4 * g]);
-#pragma unroll
-                for (int s = 0; s < 4; ++s)
-#pragma unroll
-                    for (int i = 0; i < T::MB; ++i)
-#pragma unroll
-                        for (int j = 0; j < T::NB; ++j)
-                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][s], b[j][s],
-                                                                             acc[i][j], 0, 0, 0);
-            }
-            if (more) sstore(smem + ((kt + 1) & 1) * T::STAGE);
-            __syncthreads();
-        }
-
-        // ---- epilogue: lane holds rows 4g..4g+3 of column l16 of each block.
-        const float alpha = P.alpha, beta = P.beta;
-#pragma unroll
-        for (int j = 0; j < T::NB; ++j) {
-            const int col = n0 + wn * T::TN + 16 * j + l16;
-            if (col >= N) continue;
-            float* cc = P.C + (size_t)col * P.ldc;
-#pragma unroll
-            for (int i = 0; i < T::MB; ++i) {
-                const int row = m0 + wm * T::TM + 16 * i + 4 * g;
-                const int valid = min(max(M - row, 0), 4);
-                if (valid == 0) continue;
-                f32x4 v;
-                if (beta != 0.f) {
-                    const f32x4 c = load4<VEC>(cc + row, valid);
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) v[r] = __builtin_fmaf(beta, c[r], alpha * acc[i][j][r]);
-                } else {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) v[r] = alpha * acc[i][j][r];
-                }
-                if (VEC && valid == 4) {
-                    *reinterpret_cast<f32x4*>(cc + row) = v;
-                } else {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        if (r < valid) cc[row + r] = v[r];
-                }
-            }
-        }
-    }
-};
 
 // Column bands of this many tile-columns in the launch order.
 constexpr int kGroupN = 8;
@@ -206,12 +15,6 @@ __global__ __launch_bounds__(T::NT) void k_sgemm_f32(Problem P) {
     tile_coords(id, P.tiles_m, P.tiles_n, kGroupN, tm, tn);
     F32TileRunner<T, VEC>::run(P, tm, tn, smem);
 }
-
-// ---------------------------------------------------------------- configs
-using F32_128x128 = F32Tile<128, 128, 32, 2, 2>;
-using F32_64x64 = F32Tile<64, 64, 32, 2, 2>;
-using F32_128x64 = F32Tile<128, 64, 32, 2, 2>;
-using F32_256x128 = F32Tile<256, 128, 16, 2, 2>;
 
 static const ConfigInfo kF32Configs[] = {
     {128, 128, 32, 2, 2, "f32_128x128x32_w2x2"},
@@ -231,6 +34,7 @@ static hipError_t launch_tile(const Problem& p, hipStream_t s) {
     const long long tiles = (long long)q.tiles_m * q.tiles_n;
     if (tiles <= 0) return hipSuccess;
     if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
+    (void)hipGetLastError();  // the status read below must belong to this launch
     if (vec4_ok(p))
         hipLaunchKernelGGL((k_sgemm_f32<T, true>), dim3((unsigned)tiles), dim3(T::NT), 0, s, q);
     else
@@ -239,13 +43,9 @@ static hipError_t launch_tile(const Problem& p, hipStream_t s) {
 }
 
 hipError_t launch_f32(int id, const Problem& p, hipStream_t s) {
-    switch (id) {
-        case 0: return launch_tile<F32_128x128>(p, s);
-        case 1: return launch_tile<F32_64x64>(p, s);
-        case 2: return launch_tile<F32_128x64>(p, s);
-        case 3: return launch_tile<F32_256x128>(p, s);
-        default: return hipErrorInvalidValue;
-    }
+    hipError_t e = hipErrorInvalidValue;
+    dispatch_f32(id, [&](auto tag) { e = launch_tile<typename decltype(tag)::type>(p, s); });
+    return e;
 }
 
 // ------------------------------------------------------------ naive kernel
@@ -269,6 +69,7 @@ hipError_t launch_basic(const Problem& p, hipStream_t s) {
     if (p.m <= 0 || p.n <= 0) return hipSuccess;
     dim3 grid(div_up(p.m, 64), div_up(p.n, 4));
     if (grid.y > 65535) return hipErrorInvalidValue;
+    (void)hipGetLastError();
     hipLaunchKernelGGL(k_sgemm_basic, grid, dim3(256), 0, s, p);
     return hipGetLastError();
 }
@@ -286,6 +87,7 @@ __global__ __launch_bounds__(256) void k_scale_c(Problem P) {
 
 hipError_t launch_scale_c(const Problem& p, hipStream_t s) {
     if (p.m <= 0 || p.n <= 0) return hipSuccess;
+    (void)hipGetLastError();
     if (p.n > 65535) {
         // Chunk the column range to stay within the grid's y limit.
         for (int j0 = 0; j0 < p.n; j0 += 65535) {

@@ -1,0 +1,251 @@
+// tile_x3.hpp -- the BF16x3 split-precision workgroup tile on the gfx950 BF16
+// matrix cores.  Shared by the single-problem kernels (sgemm_bf16x3.hip) and
+// the persistent grouped kernel (grouped.hip).
+//
+// C := alpha*A*B + beta*C, column-major, FP32 in and out.  No reference
+// counterpart (the reference is FP32 FFMA only); same operand conventions and
+// epilogue semantics as sgemm_f32.hip.
+//
+// Each FP32 input x is split as it is staged into LDS:
+//     hi = bf16_rne(x),  lo = bf16_rne(x - hi)      (x - hi is exact in FP32)
+// and A*B is accumulated in FP32 as  hi*hi + (hi*lo + lo*hi)  on
+// v_mfma_f32_16x16x32_bf16 (BF16 products are exact in FP32).  The small cross
+// terms go to their own accumulators, added to the main ones in the epilogue,
+// so their rounding stays 2^-8 below the main sum's.  Dropped: lo*lo and the
+// residual x - hi - lo, at most ~3*2^-16 |a||b| per product.
+//
+// Global -> LDS: 16-byte loads of FP32 into registers for k-tile t+1 during
+// the MFMAs of tile t; the split happens in registers; hi and lo land in LDS
+// k-contiguous per row ([row][k], 16-byte padded rows) so every MFMA operand
+// (8 consecutive k of one row per lane) is one ds_read_b128.  A is loaded in
+// 4x4 (m x k) sub-blocks so its transpose to k-contiguous rows needs only
+// 8-byte LDS writes.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace sgemm_amd {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+
+template <int BM_, int BN_, int BK_, int WM_, int WN_>
+struct X3Tile {
+    static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
+    static constexpr int NT = 64 * WM * WN;
+    static constexpr int TM = BM / WM, TN = BN / WN;
+    static constexpr int MB = TM / 16, NB = TN / 16;
+    static constexpr int RS = BK * 2 + 16;               // LDS row stride, bytes
+    static constexpr int A_BYTES = BM * RS;              // one of Ahi / Alo
+    static constexpr int B_BYTES = BN * RS;
+    static constexpr int STAGE = 2 * A_BYTES + 2 * B_BYTES;
+    static constexpr int LDS_BYTES = 2 * STAGE;
+    static constexpr int A_SUB = (BM / 4) * (BK / 4) / NT;  // 4x4 sub-blocks of A per thread
+    static constexpr int A_MSTEP = NT / (BK / 4);           // m4 rows per pass
+    static constexpr int B_VEC = BN * BK / 4 / NT;          // float4 of B per thread
+    static constexpr int B_NSTEP = NT / (BK / 4);
+    static_assert(BK % 32 == 0, "BK must be a multiple of 32");
+    static_assert(TM % 16 == 0 && TN % 16 == 0, "wave tile");
+    static_assert(NT % (BK / 4) == 0, "load mapping");
+    static_assert(A_SUB >= 1 && (BM / 4) * (BK / 4) % NT == 0, "A split");
+    static_assert(B_VEC >= 1 && (BN * BK / 4) % NT == 0, "B split");
+    static_assert(LDS_BYTES <= 160 * 1024, "LDS budget");
+};
+
+template <bool VEC>
+__device__ inline f32x4 x3_load4(const float* p, int valid) {
+    f32x4 v = {0.f, 0.f, 0.f, 0.f};
+    if (VEC && valid >= 4) return *reinterpret_cast<const f32x4*>(p);
+    if (valid > 0) v[0] = p[0];
+    if (valid > 1) v[1] = p[1];
+    if (valid > 2) v[2] = p[2];
+    if (valid > 3) v[3] = p[3];
+    return v;
+}
+
+// hi/lo split of four values (round to nearest even both times).
+__device__ inline void split4(f32x4 x, bf16x4_t& hi, bf16x4_t& lo) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const __bf16 h = static_cast<__bf16>(x[i]);
+        hi[i] = h;
+        lo[i] = static_cast<__bf16>(x[i] - static_cast<float>(h));
+    }
+}
+
+template <class T, bool VEC>
+struct X3TileRunner {
+    static constexpr int BM = T::BM, BN = T::BN, BK = T::BK;
+
+    __device__ static void run(const Problem& P, int tm, int tn, char* smem) {
+        const int t = threadIdx.x;
+        const int lane = t & 63, wave = t >> 6;
+        const int wm = wave % T::WM, wn = wave / T::WM;
+        const int l16 = lane & 15, g = lane >> 4;
+        const int m0 = tm * BM, n0 = tn * BN;
+        const int M = P.m, N = P.n, K = P.k;
+
+        // A: thread owns 4x4 (m x k) sub-blocks at k = 4*a_k4, m = 4*(a_m4 + r*A_MSTEP)
+        const int a_k4 = t % (BK / 4);
+        const int a_m4 = t / (BK / 4);
+        int a_valid[T::A_SUB];
+        const float* a_src[T::A_SUB];
+#pragma unroll
+        for (int r = 0; r < T::A_SUB; ++r) {
+            const int row = m0 + 4 * (a_m4 + r * T::A_MSTEP);
+            a_valid[r] = min(max(M - row, 0), 4);
+            a_src[r] = P.A + (size_t)(a_valid[r] > 0 ? row : m0) + (size_t)(4 * a_k4) * P.lda;
+        }
+        // B: float4 along k at column n
+        const int b_k = 4 * (t % (BK / 4));
+        const int b_n = t / (BK / 4);
+        const float* b_src[T::B_VEC];
+        bool b_colok[T::B_VEC];
+#pragma unroll
+        for (int r = 0; r < T::B_VEC; ++r) {
+            const int col = n0 + b_n + r * T::B_NSTEP;
+            b_colok[r] = col < N;
+            b_src[r] = P.B + (size_t)(b_colok[r] ? col : 0) * P.ldb + b_k;
+        }
+
+        f32x4 ra[T::A_SUB][4], rb[T::B_VEC];
+        auto gload = [&](int k0) {
+#pragma unroll
+            for (int r = 0; r < T::A_SUB; ++r)
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const int valid = (k0 + 4 * a_k4 + kk < K) ? a_valid[r] : 0;
+                    ra[r][kk] = x3_load4<VEC>(a_src[r] + (size_t)(k0 + kk) * P.lda, valid);
+                }
+#pragma unroll
+            for (int r = 0; r < T::B_VEC; ++r) {
+                const int valid = b_colok[r] ? min(max(K - (k0 + b_k), 0), 4) : 0;
+                rb[r] = x3_load4<VEC>(b_src[r] + k0, valid);
+            }
+        };
+        auto sstore = [&](char* stage) {
+            char* Ahi = stage;
+            char* Alo = stage + T::A_BYTES;
+            char* Bhi = stage + 2 * T::A_BYTES;
+            char* Blo = Bhi + T::B_BYTES;
+#pragma unroll
+            for (int r = 0; r < T::A_SUB; ++r)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    // row 4*m4 + i gets k = 4*a_k4 .. +3
+                    f32x4 col = {ra[r][0][i], ra[r][1][i], ra[r][2][i], ra[r][3][i]};
+                    bf16x4_t hi, lo;
+                    split4(col, hi, lo);
+                    const int off = (4 * (a_m4 + r * T::A_MSTEP) + i) * T::RS + 8 * a_k4;
+                    *reinterpret_cast<bf16x4_t*>(Ahi + off) = hi;
+                    *reinterpret_cast<bf16x4_t*>(Alo + off) = lo;
+                }
+#pragma unroll
+            for (int r = 0; r < T::B_VEC; ++r) {
+                bf16x4_t hi, lo;
+                split4(rb[r], hi, lo);
+                const int off = (b_n + r * T::B_NSTEP) * T::RS + 2 * b_k;
+                *reinterpret_cast<bf16x4_t*>(Bhi + off) = hi;
+                *reinterpret_cast<bf16x4_t*>(Blo + off) = lo;
+            }
+        };
+
+        f32x4 acc[T::MB][T::NB], crs[T::MB][T::NB];
+#pragma unroll
+        for (int i = 0; i < T::MB; ++i)
+#pragma unroll
+            for (int j = 0; j < T::NB; ++j) {
+                acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+                crs[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+            }
+
+        const int ktiles = div_up(K, BK);
+        if (ktiles > 0) {
+            gload(0);
+            sstore(smem);
+            __syncthreads();
+        }
+        for (int kt = 0; kt < ktiles; ++kt) {
+            const char* stage = smem + (kt & 1) * T::STAGE;
+            const char* Ahi = stage;
+            const char* Alo = stage + T::A_BYTES;
+            const char* Bhi = stage + 2 * T::A_BYTES;
+            const char* Blo = Bhi + T::B_BYTES;
+            const bool more = kt + 1 < ktiles;
+            if (more) gload((kt + 1) * BK);
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 32) {
+                // lane group g supplies k = kk + 8g .. kk + 8g + 7 for A and B
+                bf16x8_t ah[T::MB], al[T::MB], bh[T::NB], bl[T::NB];
+#pragma unroll
+                for (int i = 0; i < T::MB; ++i) {
+                    const int off = (wm * T::TM + 16 * i + l16) * T::RS + 2 * (kk + 8 * g);
+                    ah[i] = *reinterpret_cast<const bf16x8_t*>(Ahi + off);
+                    al[i] = *reinterpret_cast<const bf16x8_t*>(Alo + off);
+                }
+#pragma unroll
+                for (int j = 0; j < T::NB; ++j) {
+                    const int off = (wn * T::TN + 16 * j + l16) * T::RS + 2 * (kk + 8 * g);
+                    bh[j] = *reinterpret_cast<const bf16x8_t*>(Bhi + off);
+                    bl[j] = *reinterpret_cast<const bf16x8_t*>(Blo + off);
+                }
+#pragma unroll
+                for (int i = 0; i < T::MB; ++i)
+#pragma unroll
+                    for (int j = 0; j < T::NB; ++j) {
+                        crs[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], crs[i][j], 0, 0, 0);
+                        crs[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], crs[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                    }
+            }
+            if (more) sstore(smem + ((kt + 1) & 1) * T::STAGE);
+            __syncthreads();
+        }
+
+        const float alpha = P.alpha, beta = P.beta;
+#pragma unroll
+        for (int j = 0; j < T::NB; ++j) {
+            const int col = n0 + wn * T::TN + 16 * j + l16;
+            if (col >= N) continue;
+            float* cc = P.C + (size_t)col * P.ldc;
+#pragma unroll
+            for (int i = 0; i < T::MB; ++i) {
+                const int row = m0 + wm * T::TM + 16 * i + 4 * g;
+                const int valid = min(max(M - row, 0), 4);
+                if (valid == 0) continue;
+                f32x4 v;
+                const f32x4 s = acc[i][j] + crs[i][j];
+                if (beta != 0.f) {
+                    const f32x4 c = x3_load4<VEC>(cc + row, valid);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) v[r] = __builtin_fmaf(beta, c[r], alpha * s[r]);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) v[r] = alpha * s[r];
+                }
+                if (VEC && valid == 4) {
+                    *reinterpret_cast<f32x4*>(cc + row) = v;
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (r < valid) cc[row + r] = v[r];
+                }
+            }
+        }
+    }
+};
+
+// Config ids of the BF16x3 form (the order is the ABI's config id).
+using X3_128x128 = X3Tile<128, 128, 32, 2, 2>;
+using X3_64x64 = X3Tile<64, 64, 64, 2, 2>;
+
+template <class F>
+__host__ __device__ inline bool dispatch_x3(int id, F&& f) {
+    switch (id) {
+        case 0: f(TypeTag<X3_128x128>{}); return true;
+        case 1: f(TypeTag<X3_64x64>{}); return true;
+        default: return false;
+    }
+}
+
+}  // namespace sgemm_amd

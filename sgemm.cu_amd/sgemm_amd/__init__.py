@@ -34,6 +34,59 @@ STATUS = {
 }
 
 
+class Problem(ctypes.Structure):
+    """Mirror of sgemm_problem (include/sgemm.h)."""
+
+    _fields_ = [
+        ("m", ctypes.c_int), ("n", ctypes.c_int), ("k", ctypes.c_int),
+        ("alpha", ctypes.c_float), ("beta", ctypes.c_float),
+        ("A", ctypes.c_void_p), ("lda", ctypes.c_int),
+        ("B", ctypes.c_void_p), ("ldb", ctypes.c_int),
+        ("C", ctypes.c_void_p), ("ldc", ctypes.c_int),
+    ]
+
+
+EXECUTOR_GPU = 0
+EXECUTOR_MOCK = 1
+OPERANDS_DEVICE = 0
+OPERANDS_HOST = 1
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", ctypes.c_size_t), ("max_batch", ctypes.c_int),
+        ("num_slots", ctypes.c_int), ("form", ctypes.c_int), ("executor", ctypes.c_int),
+        ("mock_tflops", ctypes.c_double), ("mock_launch_us", ctypes.c_double),
+    ]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("m", ctypes.c_int), ("n", ctypes.c_int), ("k", ctypes.c_int),
+        ("alpha", ctypes.c_float), ("beta", ctypes.c_float),
+        ("A", ctypes.c_void_p), ("lda", ctypes.c_int),
+        ("B", ctypes.c_void_p), ("ldb", ctypes.c_int),
+        ("C", ctypes.c_void_p), ("ldc", ctypes.c_int),
+        ("operands", ctypes.c_int),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong),
+        ("rejected", ctypes.c_longlong), ("failed", ctypes.c_longlong),
+        ("batches", ctypes.c_longlong), ("measured", ctypes.c_longlong),
+        ("mean_batch", ctypes.c_double), ("p50_ms", ctypes.c_double),
+        ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+        ("mean_queue_ms", ctypes.c_double), ("window_s", ctypes.c_double),
+        ("tflops", ctypes.c_double), ("peak_bytes", ctypes.c_size_t),
+        ("budget_bytes", ctypes.c_size_t),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class SgemmError(RuntimeError):
     def __init__(self, status: int, where: str):
         self.status = status
@@ -81,6 +134,41 @@ def lib() -> ctypes.CDLL:
                                     ctypes.POINTER(i), ctypes.c_char_p, i]
     L.sgemm_config_info.restype = i
     L.sgemm_version.restype = ctypes.c_char_p
+    L.sgemm_hip_error_string.restype = ctypes.c_char_p
+    P = ctypes.POINTER(Problem)
+    plan_p = ctypes.c_void_p
+    for name, args in {
+        "sgemm_grouped_plan_create": [ctypes.POINTER(plan_p), i, i],
+        "sgemm_grouped_plan_set": [plan_p, i, P, vp],
+        "sgemm_grouped_plan_set_configs": [plan_p, i, P, ctypes.POINTER(i), vp],
+        "sgemm_grouped_plan_run": [plan_p, vp],
+        "sgemm_grouped_plan_capture": [plan_p, vp],
+        "sgemm_grouped_plan_replay": [plan_p, vp],
+        "sgemm_grouped_plan_info": [plan_p, ctypes.POINTER(i), ctypes.POINTER(i),
+                                    ctypes.POINTER(i), ctypes.POINTER(ctypes.c_double)],
+        "sgemm_grouped_plan_destroy": [plan_p],
+        "sgemm_grouped": [i, P, i, vp],
+        "sgemm_autotune_bucket": [i, i, i, i, i, i, vp, ctypes.POINTER(i),
+                                  ctypes.POINTER(ctypes.c_float)],
+        "sgemm_autotune_lookup": [i, i, i, i, i, ctypes.POINTER(i), ctypes.POINTER(i)],
+        "sgemm_autotune_set": [i, i, i, i, i, i],
+        "sgemm_autotune_clear": [],
+        "sgemm_autotune_size": [],
+        "sgemm_autotune_save": [ctypes.c_char_p],
+        "sgemm_autotune_load": [ctypes.c_char_p],
+        "sgemm_server_create": [ctypes.POINTER(plan_p), ctypes.POINTER(ServerConfig)],
+        "sgemm_server_submit": [plan_p, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)],
+        "sgemm_server_poll": [plan_p, ctypes.c_uint64],
+        "sgemm_server_wait": [plan_p, ctypes.c_uint64, ctypes.c_double],
+        "sgemm_server_drain": [plan_p, ctypes.c_double],
+        "sgemm_server_latency": [plan_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(ctypes.c_double)],
+        "sgemm_server_get_stats": [plan_p, ctypes.POINTER(ServerStats)],
+        "sgemm_server_reset_stats": [plan_p],
+        "sgemm_server_destroy": [plan_p],
+    }.items():
+        getattr(L, name).argtypes = args
+        getattr(L, name).restype = i
     _lib = L
     return L
 
@@ -116,6 +204,8 @@ def _f(x: float):
 
 def _check(status: int, where: str) -> None:
     if status != 0:
+        if status == 3:
+            where += f" [{lib().sgemm_hip_error_string().decode()}]"
         raise SgemmError(status, where)
 
 
@@ -185,3 +275,174 @@ def status_string(status: int) -> str:
 
 def version() -> str:
     return lib().sgemm_version().decode()
+
+
+# ------------------------------------------------------------ grouped GEMM
+def problem(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc) -> Problem:
+    return Problem(m, n, k, alpha, beta, _ptr(A), lda, _ptr(B), ldb, _ptr(C), ldc)
+
+
+def _problem_array(problems):
+    arr = (Problem * max(1, len(problems)))()
+    for idx, p in enumerate(problems):
+        arr[idx] = p
+    return arr
+
+
+class GroupedPlan:
+    """A grouped-GEMM plan: descriptors for up to `capacity` problems, one
+    persistent launch per run, optional hipGraph capture/replay."""
+
+    def __init__(self, capacity: int, form: int = FORM_F32):
+        self.handle = ctypes.c_void_p()
+        _check(lib().sgemm_grouped_plan_create(ctypes.byref(self.handle), capacity, form),
+               "sgemm_grouped_plan_create")
+
+    def set(self, problems, stream=None, configs=None) -> None:
+        arr = _problem_array(problems)
+        if configs is None:
+            _check(lib().sgemm_grouped_plan_set(self.handle, len(problems), arr, _stream(stream)),
+                   "sgemm_grouped_plan_set")
+        else:
+            cfg = (ctypes.c_int * max(1, len(configs)))(*configs)
+            _check(lib().sgemm_grouped_plan_set_configs(self.handle, len(problems), arr, cfg,
+                                                        _stream(stream)),
+                   "sgemm_grouped_plan_set_configs")
+
+    def run(self, stream=None) -> None:
+        _check(lib().sgemm_grouped_plan_run(self.handle, _stream(stream)), "sgemm_grouped_plan_run")
+
+    def capture(self, stream) -> None:
+        _check(lib().sgemm_grouped_plan_capture(self.handle, _stream(stream)),
+               "sgemm_grouped_plan_capture")
+
+    def replay(self, stream=None) -> None:
+        _check(lib().sgemm_grouped_plan_replay(self.handle, _stream(stream)),
+               "sgemm_grouped_plan_replay")
+
+    def info(self) -> dict:
+        c, t, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        f = ctypes.c_double()
+        _check(lib().sgemm_grouped_plan_info(self.handle, ctypes.byref(c), ctypes.byref(t),
+                                             ctypes.byref(g), ctypes.byref(f)),
+               "sgemm_grouped_plan_info")
+        return {"count": c.value, "total_tiles": t.value, "grid": g.value, "flops": f.value}
+
+    def destroy(self) -> None:
+        if self.handle:
+            _check(lib().sgemm_grouped_plan_destroy(self.handle), "sgemm_grouped_plan_destroy")
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().sgemm_grouped_plan_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def grouped(problems, form=FORM_F32, stream=None) -> None:
+    arr = _problem_array(problems)
+    _check(lib().sgemm_grouped(len(problems), arr, form, _stream(stream)), "sgemm_grouped")
+
+
+# --------------------------------------------------------------- autotuner
+def autotune_bucket(form, m, n, k, iters=3, grouped=False, stream=None):
+    cfg, tf = ctypes.c_int(), ctypes.c_float()
+    _check(lib().sgemm_autotune_bucket(form, int(grouped), m, n, k, iters, _stream(stream),
+                                       ctypes.byref(cfg), ctypes.byref(tf)),
+           "sgemm_autotune_bucket")
+    return cfg.value, tf.value
+
+
+def autotune_lookup(form, m, n, k, grouped=False):
+    cfg, from_table = ctypes.c_int(), ctypes.c_int()
+    _check(lib().sgemm_autotune_lookup(form, int(grouped), m, n, k, ctypes.byref(cfg),
+                                       ctypes.byref(from_table)), "sgemm_autotune_lookup")
+    return cfg.value, bool(from_table.value)
+
+
+def autotune_set(form, m, n, k, config_id, grouped=False):
+    _check(lib().sgemm_autotune_set(form, int(grouped), m, n, k, config_id), "sgemm_autotune_set")
+
+
+def autotune_clear():
+    _check(lib().sgemm_autotune_clear(), "sgemm_autotune_clear")
+
+
+def autotune_size() -> int:
+    return int(lib().sgemm_autotune_size())
+
+
+def autotune_save(path: str):
+    _check(lib().sgemm_autotune_save(path.encode()), "sgemm_autotune_save")
+
+
+def autotune_load(path: str):
+    _check(lib().sgemm_autotune_load(path.encode()), "sgemm_autotune_load")
+
+
+# ------------------------------------------------------------ serving loop
+class Server:
+    """Continuous-batching GEMM server (sgemm_server_* in include/sgemm.h)."""
+
+    def __init__(self, budget_bytes, max_batch=64, num_slots=2, form=FORM_BF16X3,
+                 executor=EXECUTOR_GPU, mock_tflops=0.0, mock_launch_us=0.0):
+        cfg = ServerConfig(budget_bytes, max_batch, num_slots, form, executor, mock_tflops,
+                           mock_launch_us)
+        self.handle = ctypes.c_void_p()
+        _check(lib().sgemm_server_create(ctypes.byref(self.handle), ctypes.byref(cfg)),
+               "sgemm_server_create")
+
+    def submit(self, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, operands=OPERANDS_DEVICE,
+               check=True):
+        r = Request(m, n, k, alpha, beta, _hptr(A), lda, _hptr(B), ldb, _hptr(C), ldc, operands)
+        rid = ctypes.c_uint64()
+        st = lib().sgemm_server_submit(self.handle, ctypes.byref(r), ctypes.byref(rid))
+        if check:
+            _check(st, "sgemm_server_submit")
+        return rid.value if st == 0 else -st
+
+    def poll(self, rid) -> int:
+        return lib().sgemm_server_poll(self.handle, rid)
+
+    def wait(self, rid, timeout=60.0) -> int:
+        return lib().sgemm_server_wait(self.handle, rid, timeout)
+
+    def drain(self, timeout=120.0) -> None:
+        _check(lib().sgemm_server_drain(self.handle, timeout), "sgemm_server_drain")
+
+    def latency(self, rid):
+        t, q = ctypes.c_double(), ctypes.c_double()
+        _check(lib().sgemm_server_latency(self.handle, rid, ctypes.byref(t), ctypes.byref(q)),
+               "sgemm_server_latency")
+        return t.value, q.value
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(lib().sgemm_server_get_stats(self.handle, ctypes.byref(st)), "sgemm_server_stats")
+        return st.as_dict()
+
+    def reset_stats(self) -> None:
+        _check(lib().sgemm_server_reset_stats(self.handle), "sgemm_server_reset_stats")
+
+    def destroy(self) -> None:
+        if self.handle:
+            _check(lib().sgemm_server_destroy(self.handle), "sgemm_server_destroy")
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().sgemm_server_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def _hptr(x):
+    """Device tensor / int pointer, or a numpy array (host operands)."""
+    if x is None:
+        return None
+    if hasattr(x, "ctypes") and hasattr(x, "dtype"):
+        return x.ctypes.data
+    return _ptr(x)

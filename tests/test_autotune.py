@@ -1,0 +1,88 @@
+"""Host-side behaviour of the autotune table (no GPU): bucketing, lookup,
+hand-set entries, JSON save / load round trip, malformed files, and the
+SGEMM_AMD_TUNE_TABLE environment hook.  Timing candidates needs a GPU and is
+covered in tests/test_grouped_gpu.py."""
+import os
+import subprocess
+import sys
+import tempfile
+
+import pytest
+
+import sgemm_amd as sa
+
+
+@pytest.fixture(autouse=True)
+def clean_table():
+    if not os.path.exists(sa.LIB_PATH):
+        sa.build()
+    sa.autotune_clear()
+    yield
+    sa.autotune_clear()
+
+
+def test_default_heuristic_is_deterministic_and_valid():
+    for form in (sa.FORM_F32, sa.FORM_BF16X3):
+        n_cfg = len(sa.configs(form))
+        for shape in [(1, 1, 1), (32, 2048, 77), (1024, 1024, 1024), (8192, 8192, 8192)]:
+            a = sa.autotune_lookup(form, *shape)
+            assert a == sa.autotune_lookup(form, *shape)
+            assert 0 <= a[0] < n_cfg and a[1] is False
+            g = sa.autotune_lookup(form, *shape, grouped=True)
+            assert g[1] is False and g[0] in ((0, 1, 2) if form == sa.FORM_F32 else (0, 1))
+
+
+def test_set_applies_to_whole_bucket_only():
+    sa.autotune_set(sa.FORM_F32, 1000, 1000, 1000, 1)
+    assert sa.autotune_lookup(sa.FORM_F32, 513, 600, 1024) == (1, True)     # same bucket
+    assert sa.autotune_lookup(sa.FORM_F32, 1025, 600, 1024)[1] is False      # next m bucket
+    assert sa.autotune_lookup(sa.FORM_F32, 1000, 1000, 1000, grouped=True)[1] is False
+    assert sa.autotune_lookup(sa.FORM_BF16X3, 1000, 1000, 1000)[1] is False
+
+
+def test_invalid_entries_rejected():
+    with pytest.raises(sa.SgemmError):
+        sa.autotune_set(sa.FORM_F32, 10, 10, 10, 99)
+    with pytest.raises(sa.SgemmError):
+        sa.autotune_set(sa.FORM_F32, 10, 10, 10, 3, grouped=True)  # not in the grouped kernel
+    with pytest.raises(sa.SgemmError):
+        sa.autotune_set(7, 10, 10, 10, 0)
+
+
+def test_save_load_roundtrip_and_bad_file():
+    sa.autotune_set(sa.FORM_F32, 64, 64, 64, 1)
+    sa.autotune_set(sa.FORM_BF16X3, 4096, 4096, 4096, 0, grouped=True)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "t.json")
+        sa.autotune_save(path)
+        import json
+        data = json.load(open(path))
+        assert len(data["entries"]) == 2
+        sa.autotune_clear()
+        assert sa.autotune_size() == 0
+        sa.autotune_load(path)
+        assert sa.autotune_size() == 2
+        assert sa.autotune_lookup(sa.FORM_F32, 60, 50, 40) == (1, True)
+        assert sa.autotune_lookup(sa.FORM_BF16X3, 4000, 3000, 2100, grouped=True) == (0, True)
+        bad = os.path.join(d, "bad.json")
+        with open(bad, "w") as f:
+            f.write('{"entries": [\n  {"form": 9, "grouped": 0}\n]}\n')
+        with pytest.raises(sa.SgemmError) as e:
+            sa.autotune_load(bad)
+        assert e.value.status == 8
+        assert sa.autotune_size() == 2  # a failed load keeps the old table
+        with pytest.raises(sa.SgemmError):
+            sa.autotune_load(os.path.join(d, "missing.json"))
+
+
+def test_environment_table_loaded_on_first_use():
+    sa.autotune_set(sa.FORM_F32, 300, 300, 300, 2)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "env.json")
+        sa.autotune_save(path)
+        code = ("import sys; sys.path.insert(0, %r); import sgemm_amd as sa; "
+                "print(sa.autotune_lookup(0, 300, 300, 300))" % os.path.dirname(sa.ROOT + "/x"))
+        env = dict(os.environ, SGEMM_AMD_TUNE_TABLE=path)
+        out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
+                             text=True, check=True).stdout.strip()
+        assert out == "(2, True)"

@@ -1,0 +1,131 @@
+"""GPU parity of the BF16x3 split-precision form against the FP64 oracle.
+
+Tolerance: the plain-FP32 bound widened by the terms the split drops,
+    |C - C_ref|_ij <= 2 * (gamma_{k+2} + 3 * 2^-16) * scale_ij,
+scale = |alpha| (|A||B|) + |beta| |C0|.  (|lo*lo| <= 2^-16|ab| and the two
+residual products <= 2^-16|ab| each.)  For k >= ~800 the added term is below
+gamma_{k+2}, so at the headline sizes the form meets the plain FP32 bound
+itself -- test_error_within_fp32_bound checks exactly that.
+Integer-valued inputs are BF16-exact (lo = 0), so those cases are bit-exact.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import sgemm_amd as sa
+from gpu_util import Case, assert_ok, check, compact, download, log, run_form, upload
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gemm_cases.json")
+N_X3_CONFIGS = 2
+
+
+def test_config_table():
+    assert len(sa.configs(sa.FORM_BF16X3)) == N_X3_CONFIGS
+
+
+@pytest.mark.parametrize("config", range(N_X3_CONFIGS))
+def test_golden_cases(config):
+    with open(GOLDEN) as f:
+        golden = json.load(f)["cases"]
+    for g in golden:
+        case = Case(g["m"], g["n"], g["k"], g["lda"], g["ldb"], g["ldc"], alpha=g["alpha"],
+                    beta=g["beta"])
+        case.A = np.array(g["A"], np.float32)
+        case.B = np.array(g["B"], np.float32)
+        case.C0 = np.array(g["C0"], np.float32)
+        out = run_form(case, sa.FORM_BF16X3, config)
+        got = compact(out, g["m"], g["n"], g["ldc"])
+        if g["kind"] == "int":
+            assert np.array_equal(got, np.array(g["expected_f32_rounded"], np.float32)), g["name"]
+        assert_ok(check(case, out, orc.bf16x3_tol_mult(g["k"])), f"{g['name']} config {config}")
+
+
+def test_split_matches_rne_definition():
+    """K = 1 and B = 1: C = hi(a) + lo(a) exactly, which pins both GPU
+    conversions to the CPU round-to-nearest-even definition."""
+    m = 4096
+    a = orc.fill("wide", m, 77)
+    # add exact ties and near-ties of the BF16 grid
+    specials = np.array([1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -(1.0 + 3 * 2.0 ** -8),
+                         1.0 + 2.0 ** -8 + 2.0 ** -20, 3.0e-39, 1.0e-30, 65504.0, 0.0],
+                        dtype=np.float32)
+    a[: specials.size] = specials
+    hi, lo = orc.bf16x3_split(a)
+    want = np.array([orc.bf16_to_f32(int(h)) + orc.bf16_to_f32(int(l)) for h, l in zip(hi, lo)],
+                    dtype=np.float32)
+    dA, dB, dC = upload(a), upload(np.ones(1, np.float32)), upload(np.zeros(m, np.float32))
+    for config in range(N_X3_CONFIGS):
+        log(f"bf16x3 split probe config {config}")
+        sa.run_config(sa.FORM_BF16X3, config, m, 1, 1, 1.0, dA, m, dB, 1, 0.0, dC, m)
+        assert np.array_equal(download(dC), want), config
+
+
+@pytest.mark.parametrize("config", range(N_X3_CONFIGS))
+def test_integer_kat_bit_exact(config):
+    for (m, n, k) in [(300, 257, 129), (129, 130, 1000), (64, 64, 8192)]:
+        c = Case(m, n, k, lda=m + 3, ldb=k + 1, ldc=m + 5, kind="int", seed=m + n + k)
+        out = run_form(c, sa.FORM_BF16X3, config)
+        ref, _ = c.reference()
+        assert np.array_equal(compact(out, m, n, c.ldc).astype(np.float64), ref), (m, n, k)
+
+
+@pytest.mark.parametrize("config", range(N_X3_CONFIGS))
+def test_edge_sweep(config):
+    rng = np.random.default_rng(99)
+    shapes = [(s, s, s) for s in range(1, 40, 3)]
+    shapes += [tuple(int(v) for v in rng.integers(1, 200, size=3)) for _ in range(40)]
+    shapes += [(127, 129, 31), (128, 128, 32), (129, 127, 33), (65, 63, 65)]
+    for (m, n, k) in shapes:
+        c = Case(m, n, k, lda=m + 1, ldb=k + 2, ldc=m + 3, seed=m + 5 * n + 7 * k)
+        out = run_form(c, sa.FORM_BF16X3, config)
+        assert_ok(check(c, out, orc.bf16x3_tol_mult(k)), f"edge {(m, n, k)} config {config}")
+
+
+def test_beta_zero_nan_c_and_k_zero():
+    c = Case(150, 70, 64, seed=4, beta=0.0, c_fill=np.nan)
+    out = run_form(c, sa.FORM_BF16X3)
+    assert not np.isnan(compact(out, c.m, c.n, c.ldc)).any()
+    assert_ok(check(c, out, orc.bf16x3_tol_mult(c.k)), "beta=0 NaN C")
+    c = Case(40, 30, 0, seed=4, alpha=2.0, beta=0.5)
+    out = run_form(c, sa.FORM_BF16X3)
+    assert np.array_equal(compact(out, 40, 30, c.ldc), np.float32(0.5) * compact(c.C0, 40, 30, c.ldc))
+
+
+@pytest.mark.parametrize("n", [1024, 2048])
+def test_wide_range_error_report(n):
+    """Config-3 style inputs (sign * 2^u, u in [-8, 8)): both forms against FP64.
+    Prints max/mean componentwise and relative errors of each form."""
+    res = {}
+    c = Case(n, n, n, kind="wide", seed=3000 + n, alpha=1.0, beta=0.0)
+    ref = c.reference()
+    for form, tol in ((sa.FORM_F32, orc.fp32_tol_mult(n)), (sa.FORM_BF16X3, orc.bf16x3_tol_mult(n))):
+        out = run_form(c, form)
+        st = check(c, out, tol, ref=ref)
+        assert_ok(st, f"wide {n} form {form}")
+        res[form] = st
+    log(f"wide-range {n}^3: f32 max/mean comp {res[0]['max_err']:.3e}/{res[0]['mean_err']:.3e} "
+        f"max/mean rel {res[0]['max_rel']:.3e}/{res[0]['mean_rel']:.3e} | bf16x3 max/mean comp "
+        f"{res[1]['max_err']:.3e}/{res[1]['mean_err']:.3e} max/mean rel "
+        f"{res[1]['max_rel']:.3e}/{res[1]['mean_rel']:.3e}")
+
+
+def test_error_within_fp32_bound_at_8192():
+    """At k = 8192 the BF16x3 result must meet the plain FP32 stated bound
+    2 gamma_{k+2} scale (sampled entries against FP64)."""
+    n = 8192
+    c = Case(n, n, n, kind="wide", seed=8192, alpha=1.0, beta=0.0)
+    out = run_form(c, sa.FORM_BF16X3)
+    rng = np.random.default_rng(1)
+    ii = rng.integers(0, n, 4096).astype(np.int32)
+    jj = rng.integers(0, n, 4096).astype(np.int32)
+    ref, scale = orc.dgemm_entries(n, 1.0, c.A, n, c.B, n, 0.0, None, n, ii, jj)
+    got = out[ii + jj.astype(np.int64) * n].astype(np.float64)
+    e = np.abs(got - ref) / scale
+    log(f"bf16x3 8192^3 wide sampled max comp err {e.max():.3e} (fp32 bound "
+        f"{orc.fp32_tol_mult(n):.3e})")
+    assert np.all(e <= orc.fp32_tol_mult(n))

@@ -1,0 +1,135 @@
+"""Serving-loop scheduling on the mock executor (no GPU): admission against
+the HBM budget, strict FIFO, continuous batching into launches, slot reuse,
+rejection of requests that can never fit, and the latency statistics.  The
+mock completes a launch after flops / mock_tflops seconds of wall time."""
+import time
+
+import numpy as np
+import pytest
+
+import sgemm_amd as sa
+
+MB = 1 << 20
+
+
+@pytest.fixture(autouse=True)
+def built():
+    import os
+    if not os.path.exists(sa.LIB_PATH):
+        sa.build()
+
+
+def _submit(srv, m, n, k, **kw):
+    # device-operand requests with dummy pointers: the mock never dereferences
+    return srv.submit(m, n, k, 1.0, 4096, m, 4096, k, 0.0, 4096, m, **kw)
+
+
+def test_config_validation():
+    for bad in [dict(budget_bytes=0), dict(budget_bytes=MB, max_batch=0),
+                dict(budget_bytes=MB, num_slots=0), dict(budget_bytes=MB, num_slots=17),
+                dict(budget_bytes=MB, form=5), dict(budget_bytes=MB, mock_tflops=0.0)]:
+        kw = dict(budget_bytes=MB, executor=sa.EXECUTOR_MOCK, mock_tflops=1.0)
+        kw.update(bad)
+        with pytest.raises(sa.SgemmError):
+            sa.Server(**kw)
+
+
+def test_all_requests_complete_and_stats():
+    srv = sa.Server(64 * MB, max_batch=8, num_slots=2, executor=sa.EXECUTOR_MOCK,
+                    mock_tflops=0.05, mock_launch_us=200)
+    rng = np.random.default_rng(0)
+    ids = []
+    for _ in range(40):
+        m, n, k = (int(v) for v in rng.integers(64, 256, 3))
+        ids.append(_submit(srv, m, n, k))
+    srv.drain(30)
+    st = srv.stats()
+    assert st["submitted"] == 40 and st["completed"] == 40 and st["failed"] == 0
+    assert st["measured"] == 40 and 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    assert st["batches"] >= 40 / 8 and st["mean_batch"] > 1.0  # batched, not one by one
+    assert st["tflops"] > 0 and st["peak_bytes"] <= 64 * MB
+    for rid in ids:
+        assert srv.poll(rid) == 0
+        total, queue = srv.latency(rid)
+        assert total >= queue >= 0
+    assert srv.poll(10 ** 9) == 7  # unknown id
+    srv.destroy()
+
+
+def test_budget_admission_and_rejection():
+    # 1 MiB budget; one 256x256x256 request needs 3 * 256 KiB = 768 KiB
+    srv = sa.Server(1 * MB, max_batch=8, num_slots=4, executor=sa.EXECUTOR_MOCK,
+                    mock_tflops=0.001, mock_launch_us=0)
+    too_big = _submit(srv, 512, 512, 512, check=False)
+    assert too_big == -5  # BUDGET_EXCEEDED
+    a = _submit(srv, 256, 256, 256)
+    b = _submit(srv, 256, 256, 256)
+    time.sleep(0.005)
+    # only one fits: the second waits although three slots are idle
+    assert srv.poll(a) == 6 and srv.poll(b) == 6
+    st = srv.stats()
+    assert st["rejected"] == 1 and st["peak_bytes"] <= 1 * MB
+    srv.drain(30)
+    ta, qa = srv.latency(a)
+    tb, qb = srv.latency(b)
+    # b was admitted only after a finished and released its footprint
+    assert qb >= ta - 1.0
+    srv.destroy()
+
+
+def test_strict_fifo_no_overtaking():
+    srv = sa.Server(1 * MB, max_batch=8, num_slots=4, executor=sa.EXECUTOR_MOCK,
+                    mock_tflops=0.002, mock_launch_us=0)
+    big1 = _submit(srv, 256, 256, 256)   # 768 KiB
+    big2 = _submit(srv, 256, 256, 256)   # must wait for big1
+    small = _submit(srv, 16, 16, 16)     # fits beside big1, but may not overtake big2
+    srv.drain(30)
+    _, q_big2 = srv.latency(big2)
+    _, q_small = srv.latency(small)
+    assert q_small >= q_big2 - 0.5
+    srv.destroy()
+
+
+def test_continuous_batching_admits_while_busy():
+    """Requests arriving while a launch is in flight go out in the next free
+    slot without waiting for the first launch to finish."""
+    srv = sa.Server(64 * MB, max_batch=4, num_slots=2, executor=sa.EXECUTOR_MOCK,
+                    mock_tflops=0.0005, mock_launch_us=0)
+    first = _submit(srv, 256, 256, 256)      # ~67 ms of mock compute
+    time.sleep(0.01)
+    second = _submit(srv, 32, 32, 32)        # tiny
+    srv.wait(second, 10)
+    assert srv.poll(first) == 6              # first still running
+    srv.drain(30)
+    t2, q2 = srv.latency(second)
+    t1, _ = srv.latency(first)
+    assert q2 < 5.0 and t2 < t1
+    srv.destroy()
+
+
+def test_host_operand_pool_fragmentation_is_safe():
+    """Host-operand requests carve A, B, C out of the pool; mixed sizes must
+    all complete and never exceed the pool."""
+    srv = sa.Server(2 * MB, max_batch=3, num_slots=3, executor=sa.EXECUTOR_MOCK,
+                    mock_tflops=0.01, mock_launch_us=50)
+    rng = np.random.default_rng(4)
+    bufs = []
+    for _ in range(30):
+        m, n, k = (int(v) for v in rng.integers(8, 200, 3))
+        A = np.zeros(m * k, np.float32)
+        B = np.zeros(k * n, np.float32)
+        C = np.zeros(m * n, np.float32)
+        bufs.append((A, B, C))
+        srv.submit(m, n, k, 1.0, A, m, B, k, 0.0, C, m, operands=sa.OPERANDS_HOST)
+    srv.drain(30)
+    st = srv.stats()
+    assert st["completed"] == 30 and st["peak_bytes"] <= 2 * MB
+    srv.destroy()
+
+
+def test_invalid_request_rejected():
+    srv = sa.Server(MB, executor=sa.EXECUTOR_MOCK, mock_tflops=1.0)
+    assert srv.submit(4, 4, 4, 1.0, 4096, 3, 4096, 4, 0.0, 4096, 4, check=False) == -1
+    assert srv.submit(4, 4, 4, 1.0, 4096, 4, 4096, 4, 0.0, 4096, 4, operands=7,
+                      check=False) == -1
+    srv.destroy()
