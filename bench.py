@@ -41,7 +41,33 @@ def parse():
     ap.add_argument("--no-error", action="store_true", help="skip the FP64 error report")
     ap.add_argument("--cpu-cols", type=int, default=48,
                     help="output columns of the CPU baseline sample")
-    return ap.parse_args()
+    ap.add_argument("--no-grouped", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--group-count", type=int, default=512)
+    ap.add_argument("--serve-seconds", type=float, default=2.0,
+                    help="arrival window per swept rate")
+    ap.add_argument("--serve-rates", type=str, default="0.5,0.9",
+                    help="arrival rates as fractions of the measured closed-loop capacity")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="time only the plain FP32 kernel (profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device-to-device copy (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.plain_only:
+        args.no_bf16x3 = args.no_error = args.no_cpu = args.no_grouped = args.no_serve = True
+    return args
+
+
+def copy_calibration(steps):
+    """A copy kernel of known size: 1 GiB read + 1 GiB written per step, the
+    yardstick for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 1 << 28
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    ms = time_form(lambda: dst.copy_(src), steps, 2, torch.cuda.current_stream())
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "ms": ms,
+                      "GBps": 2 * 4 * n / (ms * 1e-3) / 1e9}), flush=True)
 
 
 def time_form(fn, steps, warmup, stream):
@@ -92,6 +118,133 @@ def cpu_baseline(size: int, cols: int):
                       f"alpha=1.5 beta=0.5, {dt:.1f} s"}
 
 
+def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0):
+    """512 independent problems, M, N, K log-uniform integers in [32, 2048],
+    U[-1, 1), alpha=1, beta=0: one persistent launch replayed from a hipGraph,
+    against the same problems as back-to-back single launches (same configs,
+    also graph-captured so no host overhead is timed)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    shapes = np.floor(np.exp(rng.uniform(np.log(32), np.log(2049), size=(count, 3)))).astype(int)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed + 1)
+    mats, probs = [], []
+    flops = 0.0
+    for m, n, k in shapes.tolist():
+        A = torch.rand(m * k, device="cuda", generator=g) * 2 - 1
+        B = torch.rand(k * n, device="cuda", generator=g) * 2 - 1
+        C = torch.zeros(m * n, device="cuda")
+        mats.append((A, B, C))
+        probs.append(sa.problem(m, n, k, 1.0, A, m, B, k, 0.0, C, m))
+        flops += 2.0 * m * n * k
+    # autotune every size bucket present (grouped-capable configs only)
+    t0 = time.perf_counter()
+    tuned = {}
+    for m, n, k in shapes.tolist():
+        key = tuple(int(v - 1).bit_length() for v in (m, n, k))
+        if key not in tuned:
+            tuned[key] = sa.autotune_bucket(form, m, n, k, iters=2, grouped=True, stream=stream)[0]
+    tune_s = time.perf_counter() - t0
+    cfgs = [sa.autotune_lookup(form, m, n, k, grouped=True)[0] for m, n, k in shapes.tolist()]
+    torch.cuda.synchronize()
+    gs = torch.cuda.Stream()  # stream capture needs a stream other than the null stream
+    plan = sa.GroupedPlan(count, form)
+    plan.set(probs, gs)
+    plan.capture(gs)
+    ms_group = time_form(lambda: plan.replay(gs), steps, warmup, gs)
+    # the same problems and configs as single launches, captured in one graph
+    loop_graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(stream)
+    with torch.cuda.stream(side):
+        for (m, n, k), (A, B, C), c in zip(shapes.tolist(), mats, cfgs):  # warm-up
+            sa.run_config(form, c, m, n, k, 1.0, A, m, B, k, 0.0, C, m, side)
+    stream.wait_stream(side)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(loop_graph):
+        cs = torch.cuda.current_stream()
+        for (m, n, k), (A, B, C), c in zip(shapes.tolist(), mats, cfgs):
+            sa.run_config(form, c, m, n, k, 1.0, A, m, B, k, 0.0, C, m, cs)
+    ms_loop = time_form(loop_graph.replay, steps, warmup, stream)
+    info = plan.info()
+    plan.destroy()
+    return {"problems": count, "flops": flops, "buckets_tuned": len(tuned),
+            "tune_s": tune_s, "ms_grouped": ms_group,
+            "tflops_grouped": flops / (ms_group * 1e-3) / 1e12,
+            "ms_single_loop": ms_loop, "tflops_single_loop": flops / (ms_loop * 1e-3) / 1e12,
+            "speedup_vs_loop": ms_loop / ms_group, "tiles": info["total_tiles"],
+            "grid": info["grid"]}
+
+
+def serving_leg(sa, torch, rates, window_s, seed=0):
+    """Poisson arrivals of BF16x3 requests, sizes 70% [128,512], 25%
+    [1024,2048], 5% [4096,8192] (each of M, N, K uniform in the bucket);
+    operands resident in HBM (sub-blocks of two 8192^2 panels), 64 GB budget."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    side = 8192
+    Apool = torch.rand(side * side, device="cuda") * 2 - 1
+    Bpool = torch.rand(side * side, device="cuda") * 2 - 1
+    arena = torch.zeros(2 << 30, device="cuda")   # 8 GiB ring for results
+    ring = [0]
+
+    def draw():
+        r = rng.random()
+        lo, hi = (128, 512) if r < 0.70 else (1024, 2048) if r < 0.95 else (4096, 8192)
+        return tuple(int(v) for v in rng.integers(lo, hi + 1, 3))
+
+    def c_slice(m, n):
+        if ring[0] + m * n > arena.numel():
+            ring[0] = 0
+        p = arena.data_ptr() + 4 * ring[0]
+        ring[0] += (m * n + 63) // 64 * 64
+        return p
+
+    srv = sa.Server(64 << 30, max_batch=32, num_slots=3, form=sa.FORM_BF16X3)
+
+    def submit(m, n, k):
+        return srv.submit(m, n, k, 1.0, Apool, side, Bpool, side, 0.0, c_slice(m, n), m)
+
+    # closed-loop capacity: a burst of requests, all queued at once
+    for _ in range(50):
+        submit(*draw())
+    srv.drain(120)
+    srv.reset_stats()
+    burst = [draw() for _ in range(600)]
+    for s in burst:
+        submit(*s)
+    srv.drain(300)
+    cap = srv.stats()
+    capacity = cap["completed"] / cap["window_s"]
+    out = {"capacity_req_s": capacity, "capacity_tflops": cap["tflops"], "budget_gb": 64,
+           "max_batch": 32, "slots": 3, "rates": []}
+    for frac in rates:
+        lam = frac * capacity
+        n_req = max(50, int(lam * window_s))
+        gaps = rng.exponential(1.0 / lam, size=n_req)
+        sizes = [draw() for _ in range(n_req)]
+        srv.reset_stats()
+        t_start = time.perf_counter()
+        t_next = t_start
+        for gap, s in zip(gaps, sizes):
+            t_next += gap
+            while time.perf_counter() < t_next:
+                pass
+            submit(*s)
+        srv.drain(300)
+        st = srv.stats()
+        out["rates"].append({"fraction_of_capacity": frac, "rate_req_s": lam,
+                             "requests": st["measured"], "p50_ms": st["p50_ms"],
+                             "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+                             "mean_queue_ms": st["mean_queue_ms"], "tflops": st["tflops"],
+                             "mean_batch": st["mean_batch"],
+                             "peak_gb": st["peak_bytes"] / 2 ** 30})
+    srv.destroy()
+    del Apool, Bpool, arena
+    torch.cuda.empty_cache()
+    return out
+
+
 def load_traffic(size: int):
     path = os.path.join(ROOT, "profiles", f"traffic_sgemm_f32_{size}.json")
     if not os.path.exists(path):
@@ -111,6 +264,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration:
+        copy_calibration(args.steps)
+        return
     sa.lib()
     torch.manual_seed(0)
     n = args.size
@@ -165,6 +321,20 @@ def main():
         del ref, scale, Aw, Bw, Cw
         torch.cuda.empty_cache()
 
+    grouped = None
+    if not args.no_grouped:
+        grouped = {}
+        for form, name in ((sa.FORM_F32, "f32"), (sa.FORM_BF16X3, "bf16x3")):
+            print(f"bench: grouped {name}", file=sys.stderr, flush=True)
+            grouped[name] = grouped_leg(sa, torch, form, args.group_count, args.steps,
+                                        args.warmup, stream)
+
+    serving = None
+    if not args.no_serve:
+        print("bench: serving", file=sys.stderr, flush=True)
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        serving = serving_leg(sa, torch, rates, args.serve_seconds)
+
     achieved = tflops
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
@@ -191,6 +361,8 @@ def main():
         "config": {"workload": f"sgemm column-major M=N=K={n}, lda=ldb=ldc={n}, alpha=1.5, "
                                f"beta=0.5", "m": n, "n": n, "k": n},
         "forms": forms,
+        "grouped": grouped,
+        "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
