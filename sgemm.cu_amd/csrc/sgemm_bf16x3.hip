@@ -16,8 +16,11 @@ __global__ __launch_bounds__(T::NT) void k_sgemm_bf16x3(Problem P) {
 }
 
 static const ConfigInfo kBf16x3Configs[] = {
-    {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2"},
-    {64, 64, 64, 2, 2, "bf16x3_64x64x64_w2x2"},
+    {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m16_dual"},
+    {64, 64, 64, 2, 2, "bf16x3_64x64x64_w2x2_m16_dual"},
+    {256, 256, 32, 2, 4, "bf16x3_256x256x32_w2x4_m32_single"},
+    {256, 128, 32, 2, 2, "bf16x3_256x128x32_w2x2_m32_single"},
+    {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m32_dual"},
 };
 
 int bf16x3_config_count() { return (int)(sizeof(kBf16x3Configs) / sizeof(kBf16x3Configs[0])); }

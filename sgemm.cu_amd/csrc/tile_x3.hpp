@@ -22,6 +22,8 @@
 // 8-byte LDS writes.
 #pragma once
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace sgemm_amd {
@@ -29,12 +31,21 @@ namespace sgemm_amd {
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 
-template <int BM_, int BN_, int BK_, int WM_, int WN_>
+// BM x BN workgroup tile, BK k-tile, WM x WN waves.  MF selects the MFMA:
+// 16 -> v_mfma_f32_16x16x32_bf16 (16x16 blocks, 4 accumulators per lane),
+// 32 -> v_mfma_f32_32x32x16_bf16 (32x32 blocks, 16 accumulators per lane).
+// DUAL keeps hi*lo + lo*hi in separate accumulators (added in the epilogue);
+// otherwise all three products go into one set, which halves the accumulator
+// registers and so allows larger wave tiles.
+template <int BM_, int BN_, int BK_, int WM_, int WN_, int MF_ = 16, bool DUAL_ = true>
 struct X3Tile {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
+    static constexpr int MF = MF_;
+    static constexpr bool DUAL = DUAL_;
     static constexpr int NT = 64 * WM * WN;
     static constexpr int TM = BM / WM, TN = BN / WN;
-    static constexpr int MB = TM / 16, NB = TN / 16;
+    static constexpr int MB = TM / MF, NB = TN / MF;     // MFMA blocks per wave
+    static constexpr int KSTEP = MF == 16 ? 32 : 16;     // k per MFMA
     static constexpr int RS = BK * 2 + 16;               // LDS row stride, bytes
     static constexpr int A_BYTES = BM * RS;              // one of Ahi / Alo
     static constexpr int B_BYTES = BN * RS;
@@ -44,8 +55,9 @@ struct X3Tile {
     static constexpr int A_MSTEP = NT / (BK / 4);           // m4 rows per pass
     static constexpr int B_VEC = BN * BK / 4 / NT;          // float4 of B per thread
     static constexpr int B_NSTEP = NT / (BK / 4);
+    static_assert(MF == 16 || MF == 32, "MFMA shape");
     static_assert(BK % 32 == 0, "BK must be a multiple of 32");
-    static_assert(TM % 16 == 0 && TN % 16 == 0, "wave tile");
+    static_assert(TM % MF == 0 && TN % MF == 0, "wave tile");
     static_assert(NT % (BK / 4) == 0, "load mapping");
     static_assert(A_SUB >= 1 && (BM / 4) * (BK / 4) % NT == 0, "A split");
     static_assert(B_VEC >= 1 && (BN * BK / 4) % NT == 0, "B split");
@@ -81,7 +93,6 @@ struct X3TileRunner {
         const int t = threadIdx.x;
         const int lane = t & 63, wave = t >> 6;
         const int wm = wave % T::WM, wn = wave / T::WM;
-        const int l16 = lane & 15, g = lane >> 4;
         const int m0 = tm * BM, n0 = tn * BN;
         const int M = P.m, N = P.n, K = P.k;
 
@@ -150,14 +161,20 @@ struct X3TileRunner {
             }
         };
 
-        f32x4 acc[T::MB][T::NB], crs[T::MB][T::NB];
+        using Acc = typename std::conditional<T::MF == 16, f32x4, f32x16>::type;
+        constexpr int NCRS = T::DUAL ? 1 : 0;
+        Acc acc[T::MB][T::NB], crs[NCRS ? T::MB : 1][NCRS ? T::NB : 1];
 #pragma unroll
         for (int i = 0; i < T::MB; ++i)
 #pragma unroll
             for (int j = 0; j < T::NB; ++j) {
-                acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-                crs[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+                acc[i][j] = Acc{};
+                if constexpr (T::DUAL) crs[i][j] = Acc{};
             }
+
+        // Lane layout of one MFMA operand: row (lane % MF) supplies 8
+        // consecutive k at kk + 8 * (lane / MF).
+        const int lrow = lane % T::MF, lk = 8 * (lane / T::MF);
 
         const int ktiles = div_up(K, BK);
         if (ktiles > 0) {
@@ -174,18 +191,17 @@ struct X3TileRunner {
             const bool more = kt + 1 < ktiles;
             if (more) gload((kt + 1) * BK);
 #pragma unroll
-            for (int kk = 0; kk < BK; kk += 32) {
-                // lane group g supplies k = kk + 8g .. kk + 8g + 7 for A and B
+            for (int kk = 0; kk < BK; kk += T::KSTEP) {
                 bf16x8_t ah[T::MB], al[T::MB], bh[T::NB], bl[T::NB];
 #pragma unroll
                 for (int i = 0; i < T::MB; ++i) {
-                    const int off = (wm * T::TM + 16 * i + l16) * T::RS + 2 * (kk + 8 * g);
+                    const int off = (wm * T::TM + T::MF * i + lrow) * T::RS + 2 * (kk + lk);
                     ah[i] = *reinterpret_cast<const bf16x8_t*>(Ahi + off);
                     al[i] = *reinterpret_cast<const bf16x8_t*>(Alo + off);
                 }
 #pragma unroll
                 for (int j = 0; j < T::NB; ++j) {
-                    const int off = (wn * T::TN + 16 * j + l16) * T::RS + 2 * (kk + 8 * g);
+                    const int off = (wn * T::TN + T::MF * j + lrow) * T::RS + 2 * (kk + lk);
                     bh[j] = *reinterpret_cast<const bf16x8_t*>(Bhi + off);
                     bl[j] = *reinterpret_cast<const bf16x8_t*>(Blo + off);
                 }
@@ -193,44 +209,71 @@ struct X3TileRunner {
                 for (int i = 0; i < T::MB; ++i)
 #pragma unroll
                     for (int j = 0; j < T::NB; ++j) {
-                        crs[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], crs[i][j], 0, 0, 0);
-                        crs[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], crs[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                        if constexpr (T::MF == 16) {
+                            if constexpr (T::DUAL) {
+                                crs[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], crs[i][j], 0, 0, 0);
+                                crs[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], crs[i][j], 0, 0, 0);
+                            } else {
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                            }
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                        } else {
+                            if constexpr (T::DUAL) {
+                                crs[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], crs[i][j], 0, 0, 0);
+                                crs[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], crs[i][j], 0, 0, 0);
+                            } else {
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                            }
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                        }
                     }
             }
             if (more) sstore(smem + ((kt + 1) & 1) * T::STAGE);
             __syncthreads();
         }
 
+        // Epilogue.  16x16 block: lane holds rows 4*(lane/16) + 0..3 of column
+        // lane % 16.  32x32 block: register group q (4 registers) holds rows
+        // 8q + 4*(lane/32) + 0..3 of column lane % 32.
         const float alpha = P.alpha, beta = P.beta;
+        constexpr int GROUPS = T::MF == 16 ? 1 : 4;
 #pragma unroll
         for (int j = 0; j < T::NB; ++j) {
-            const int col = n0 + wn * T::TN + 16 * j + l16;
+            const int col = n0 + wn * T::TN + T::MF * j + lrow;
             if (col >= N) continue;
             float* cc = P.C + (size_t)col * P.ldc;
 #pragma unroll
-            for (int i = 0; i < T::MB; ++i) {
-                const int row = m0 + wm * T::TM + 16 * i + 4 * g;
-                const int valid = min(max(M - row, 0), 4);
-                if (valid == 0) continue;
-                f32x4 v;
-                const f32x4 s = acc[i][j] + crs[i][j];
-                if (beta != 0.f) {
-                    const f32x4 c = x3_load4<VEC>(cc + row, valid);
+            for (int i = 0; i < T::MB; ++i)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) v[r] = __builtin_fmaf(beta, c[r], alpha * s[r]);
-                } else {
+                for (int q = 0; q < GROUPS; ++q) {
+                    const int row = m0 + wm * T::TM + T::MF * i + 8 * q + 4 * (lane / T::MF);
+                    const int valid = min(max(M - row, 0), 4);
+                    if (valid == 0) continue;
+                    f32x4 s;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) v[r] = alpha * s[r];
+                    for (int r = 0; r < 4; ++r) {
+                        s[r] = acc[i][j][4 * q + r];
+                        if constexpr (T::DUAL) s[r] += crs[i][j][4 * q + r];
+                    }
+                    f32x4 v;
+                    if (beta != 0.f) {
+                        const f32x4 c = x3_load4<VEC>(cc + row, valid);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) v[r] = __builtin_fmaf(beta, c[r], alpha * s[r]);
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) v[r] = alpha * s[r];
+                    }
+                    if (VEC && valid == 4) {
+                        *reinterpret_cast<f32x4*>(cc + row) = v;
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            if (r < valid) cc[row + r] = v[r];
+                    }
                 }
-                if (VEC && valid == 4) {
-                    *reinterpret_cast<f32x4*>(cc + row) = v;
-                } else {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        if (r < valid) cc[row + r] = v[r];
-                }
-            }
         }
     }
 };
@@ -238,12 +281,18 @@ struct X3TileRunner {
 // Config ids of the BF16x3 form (the order is the ABI's config id).
 using X3_128x128 = X3Tile<128, 128, 32, 2, 2>;
 using X3_64x64 = X3Tile<64, 64, 64, 2, 2>;
+using X3_256x256_w8 = X3Tile<256, 256, 32, 2, 4, 32, false>;
+using X3_256x128_w4 = X3Tile<256, 128, 32, 2, 2, 32, false>;
+using X3_128x128_m32 = X3Tile<128, 128, 32, 2, 2, 32, true>;
 
 template <class F>
 __host__ __device__ inline bool dispatch_x3(int id, F&& f) {
     switch (id) {
         case 0: f(TypeTag<X3_128x128>{}); return true;
         case 1: f(TypeTag<X3_64x64>{}); return true;
+        case 2: f(TypeTag<X3_256x256_w8>{}); return true;
+        case 3: f(TypeTag<X3_256x128_w4>{}); return true;
+        case 4: f(TypeTag<X3_128x128_m32>{}); return true;
         default: return false;
     }
 }
