@@ -72,6 +72,7 @@ int run(int form, int config_id, int m, int n, int k, const float* alpha, const 
     if (form != SGEMM_FORM_F32 && form != SGEMM_FORM_BF16X3) return SGEMM_STATUS_NOT_SUPPORTED;
     if (config_id >= config_count(form)) return SGEMM_STATUS_NOT_SUPPORTED;
     if (m == 0 || n == 0) return SGEMM_STATUS_SUCCESS;
+    if (lda > kMaxLd || ldb > kMaxLd) return SGEMM_STATUS_NOT_SUPPORTED;
     Problem p = make_problem(m, n, k, *alpha, A, lda, B, ldb, *beta, C, ldc);
     // No product term: C := beta*C (BLAS: A and B are not referenced).
     if (k == 0 || *alpha == 0.f) return hip_status(launch_scale_c(p, stream));

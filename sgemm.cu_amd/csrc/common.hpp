@@ -36,6 +36,40 @@ static_assert(sizeof(Problem) % 8 == 0, "descriptor packing");
 
 __host__ __device__ constexpr int div_up(int a, int b) { return (a + b - 1) / b; }
 
+// Elements a column-major (rows x cols, ld) operand spans.
+__host__ __device__ inline long long extent(int rows, int cols, int ld) {
+    return (rows <= 0 || cols <= 0) ? 0 : (long long)ld * (cols - 1) + rows;
+}
+
+// Leading dimensions above this would overflow a tile's 32-bit buffer
+// offsets (up to 256 columns of B per tile); the tiled kernels reject them.
+constexpr int kMaxLd = 1 << 21;
+
+// Buffer resource over `bytes` bytes from `base` (gfx9-family word 3).  Loads
+// through it at offsets >= bytes return 0 and never fault, so edge tiles need
+// no branches around their loads.
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base, long long bytes) {
+    const unsigned n = bytes <= 0 ? 0u : bytes >= 0xffffffffLL ? 0xffffffffu : (unsigned)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)n,
+                                             0x00020000);
+}
+
+// Four consecutive floats at byte offset `off` of a buffer.  VEC: one 16-byte
+// load (offset 16-byte aligned); otherwise four 4-byte loads.
+template <bool VEC>
+__device__ inline f32x4 bload4(__amdgpu_buffer_rsrc_t r, int off) {
+    if constexpr (VEC) {
+        return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    } else {
+        f32x4 v;
+        v[0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+        v[1] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off + 4, 0, 0));
+        v[2] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off + 8, 0, 0));
+        v[3] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off + 12, 0, 0));
+        return v;
+    }
+}
+
 // Number of XCDs the dispatcher round-robins workgroups over (MI355X: 8).
 constexpr int kNumXcd = 8;
 

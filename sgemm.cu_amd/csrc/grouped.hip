@@ -138,6 +138,7 @@ int validate_problem(const sgemm_problem& q) {
     if (q.lda < std::max(1, q.m) || q.ldb < std::max(1, q.k) || q.ldc < std::max(1, q.m))
         return SGEMM_STATUS_INVALID_VALUE;
     if (q.m > 0 && q.n > 0) {
+        if (q.lda > kMaxLd || q.ldb > kMaxLd) return SGEMM_STATUS_NOT_SUPPORTED;
         if (q.C == nullptr) return SGEMM_STATUS_INVALID_VALUE;
         if (q.k > 0 && q.alpha != 0.f && (q.A == nullptr || q.B == nullptr))
             return SGEMM_STATUS_INVALID_VALUE;

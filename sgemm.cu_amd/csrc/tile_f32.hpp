@@ -78,48 +78,52 @@ struct F32TileRunner {
         const int m0 = tm * BM, n0 = tn * BN;
         const int M = P.m, N = P.n, K = P.k;
 
-        // ---- per-thread global load mapping (fixed across k-tiles)
+        // ---- per-thread global load mapping (fixed across k-tiles).  Loads
+        // go through buffer resources rebased per k-tile: rows >= M or
+        // columns >= N read either 0 (past the operand's extent) or
+        // in-extent values that only reach C rows/columns that are never
+        // stored; k >= K is zeroed explicitly on both operands.
         const int a_m = 4 * (t % (BM / 4));  // row inside the tile
         const int a_k = t / (BM / 4);        // first k row inside the tile
-        const int a_valid = min(max(M - (m0 + a_m), 0), 4);
-        const float* a_src = P.A + (size_t)(m0 + (a_valid > 0 ? a_m : 0)) + (size_t)a_k * P.lda;
-        const size_t a_kstride = (size_t)T::A_KSTEP * P.lda;
-
         const int b_k = 4 * (t % (BK / 4));
         const int b_n = t / (BK / 4);
-        const float* b_src[T::B_VEC];
-        bool b_colok[T::B_VEC];
+        int a_off[T::A_VEC], b_off[T::B_VEC];
 #pragma unroll
-        for (int r = 0; r < T::B_VEC; ++r) {
-            const int col = n0 + b_n + r * T::B_NSTEP;
-            b_colok[r] = col < N;
-            b_src[r] = P.B + (size_t)(b_colok[r] ? col : 0) * P.ldb + b_k;
-        }
+        for (int r = 0; r < T::A_VEC; ++r) a_off[r] = 4 * (a_m + (a_k + r * T::A_KSTEP) * P.lda);
+#pragma unroll
+        for (int r = 0; r < T::B_VEC; ++r) b_off[r] = 4 * (b_k + (b_n + r * T::B_NSTEP) * P.ldb);
+        const long long a_ext = extent(M, K, P.lda), b_ext = extent(K, N, P.ldb);
 
+        // Raw loads land in ra/rb; the k >= K masking happens in sstore, after
+        // the MFMAs of the current tile, so nothing waits on the loads early.
         f32x4 ra[T::A_VEC], rb[T::B_VEC];
 
         auto gload = [&](int k0) {
+            const long long a_base = m0 + (long long)k0 * P.lda;
+            const long long b_base = k0 + (long long)n0 * P.ldb;
+            const auto ar = make_rsrc(P.A + a_base, 4 * (a_ext - a_base));
+            const auto br = make_rsrc(P.B + b_base, 4 * (b_ext - b_base));
 #pragma unroll
-            for (int r = 0; r < T::A_VEC; ++r) {
-                const int kk = k0 + a_k + r * T::A_KSTEP;
-                const int valid = kk < K ? a_valid : 0;
-                ra[r] = load4<VEC>(a_src + (size_t)k0 * P.lda + r * a_kstride, valid);
-            }
+            for (int r = 0; r < T::A_VEC; ++r) ra[r] = bload4<VEC>(ar, a_off[r]);
 #pragma unroll
-            for (int r = 0; r < T::B_VEC; ++r) {
-                const int valid = b_colok[r] ? min(max(K - (k0 + b_k), 0), 4) : 0;
-                rb[r] = load4<VEC>(b_src[r] + k0, valid);
-            }
+            for (int r = 0; r < T::B_VEC; ++r) rb[r] = bload4<VEC>(br, b_off[r]);
         };
-        auto sstore = [&](float* stage) {
+        auto sstore = [&](float* stage, int k0) {
             float* As = stage;
             float* Bs = stage + BK * T::SA;
 #pragma unroll
-            for (int r = 0; r < T::A_VEC; ++r)
-                *reinterpret_cast<f32x4*>(&As[(a_k + r * T::A_KSTEP) * T::SA + a_m]) = ra[r];
+            for (int r = 0; r < T::A_VEC; ++r) {
+                const f32x4 v = (k0 + a_k + r * T::A_KSTEP < K) ? ra[r] : f32x4{0.f, 0.f, 0.f, 0.f};
+                *reinterpret_cast<f32x4*>(&As[(a_k + r * T::A_KSTEP) * T::SA + a_m]) = v;
+            }
+            const int kv = K - (k0 + b_k);  // valid k in each B float4
 #pragma unroll
-            for (int r = 0; r < T::B_VEC; ++r)
-                *reinterpret_cast<f32x4*>(&Bs[(b_n + r * T::B_NSTEP) * T::SB + b_k]) = rb[r];
+            for (int r = 0; r < T::B_VEC; ++r) {
+                f32x4 v;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[e] = e < kv ? rb[r][e] : 0.f;
+                *reinterpret_cast<f32x4*>(&Bs[(b_n + r * T::B_NSTEP) * T::SB + b_k]) = v;
+            }
         };
 
         f32x4 acc[T::MB][T::NB];
@@ -131,7 +135,7 @@ struct F32TileRunner {
         const int ktiles = div_up(K, BK);
         if (ktiles > 0) {
             gload(0);
-            sstore(smem);
+            sstore(smem, 0);
             __syncthreads();
         }
         for (int kt = 0; kt < ktiles; ++kt) {
@@ -163,7 +167,7 @@ struct F32TileRunner {
                             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][s], b[j][s],
                                                                              acc[i][j], 0, 0, 0);
             }
-            if (more) sstore(smem + ((kt + 1) & 1) * T::STAGE);
+            if (more) sstore(smem + ((kt + 1) & 1) * T::STAGE, (kt + 1) * BK);
             __syncthreads();
         }
 

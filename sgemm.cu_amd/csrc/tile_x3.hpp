@@ -96,65 +96,65 @@ struct X3TileRunner {
         const int m0 = tm * BM, n0 = tn * BN;
         const int M = P.m, N = P.n, K = P.k;
 
-        // A: thread owns 4x4 (m x k) sub-blocks at k = 4*a_k4, m = 4*(a_m4 + r*A_MSTEP)
+        // A: thread owns 4x4 (m x k) sub-blocks at k = 4*a_k4, m = 4*(a_m4 + r*A_MSTEP).
+        // Loads go through buffer resources rebased per k-tile (see
+        // tile_f32.hpp): out-of-extent reads are 0, rows >= M / columns >= N
+        // never reach a stored C element, k >= K is zeroed explicitly.
         const int a_k4 = t % (BK / 4);
         const int a_m4 = t / (BK / 4);
-        int a_valid[T::A_SUB];
-        const float* a_src[T::A_SUB];
-#pragma unroll
-        for (int r = 0; r < T::A_SUB; ++r) {
-            const int row = m0 + 4 * (a_m4 + r * T::A_MSTEP);
-            a_valid[r] = min(max(M - row, 0), 4);
-            a_src[r] = P.A + (size_t)(a_valid[r] > 0 ? row : m0) + (size_t)(4 * a_k4) * P.lda;
-        }
-        // B: float4 along k at column n
         const int b_k = 4 * (t % (BK / 4));
         const int b_n = t / (BK / 4);
-        const float* b_src[T::B_VEC];
-        bool b_colok[T::B_VEC];
+        int a_off[T::A_SUB], b_off[T::B_VEC];
 #pragma unroll
-        for (int r = 0; r < T::B_VEC; ++r) {
-            const int col = n0 + b_n + r * T::B_NSTEP;
-            b_colok[r] = col < N;
-            b_src[r] = P.B + (size_t)(b_colok[r] ? col : 0) * P.ldb + b_k;
-        }
+        for (int r = 0; r < T::A_SUB; ++r)
+            a_off[r] = 4 * (4 * (a_m4 + r * T::A_MSTEP) + 4 * a_k4 * P.lda);
+#pragma unroll
+        for (int r = 0; r < T::B_VEC; ++r) b_off[r] = 4 * (b_k + (b_n + r * T::B_NSTEP) * P.ldb);
+        const long long a_ext = extent(M, K, P.lda), b_ext = extent(K, N, P.ldb);
 
+        // Raw loads land in ra/rb; k >= K masking and the split happen in
+        // sstore, after the MFMAs of the current tile.
         f32x4 ra[T::A_SUB][4], rb[T::B_VEC];
         auto gload = [&](int k0) {
+            const long long a_base = m0 + (long long)k0 * P.lda;
+            const long long b_base = k0 + (long long)n0 * P.ldb;
+            const auto ar = make_rsrc(P.A + a_base, 4 * (a_ext - a_base));
+            const auto br = make_rsrc(P.B + b_base, 4 * (b_ext - b_base));
 #pragma unroll
             for (int r = 0; r < T::A_SUB; ++r)
 #pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const int valid = (k0 + 4 * a_k4 + kk < K) ? a_valid[r] : 0;
-                    ra[r][kk] = x3_load4<VEC>(a_src[r] + (size_t)(k0 + kk) * P.lda, valid);
-                }
+                for (int kk = 0; kk < 4; ++kk) ra[r][kk] = bload4<VEC>(ar, a_off[r] + 4 * kk * P.lda);
 #pragma unroll
-            for (int r = 0; r < T::B_VEC; ++r) {
-                const int valid = b_colok[r] ? min(max(K - (k0 + b_k), 0), 4) : 0;
-                rb[r] = x3_load4<VEC>(b_src[r] + k0, valid);
-            }
+            for (int r = 0; r < T::B_VEC; ++r) rb[r] = bload4<VEC>(br, b_off[r]);
         };
-        auto sstore = [&](char* stage) {
+        auto sstore = [&](char* stage, int k0) {
             char* Ahi = stage;
             char* Alo = stage + T::A_BYTES;
             char* Bhi = stage + 2 * T::A_BYTES;
             char* Blo = Bhi + T::B_BYTES;
+            const int ka = K - (k0 + 4 * a_k4);  // valid k in this A sub-block
 #pragma unroll
             for (int r = 0; r < T::A_SUB; ++r)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     // row 4*m4 + i gets k = 4*a_k4 .. +3
-                    f32x4 col = {ra[r][0][i], ra[r][1][i], ra[r][2][i], ra[r][3][i]};
+                    f32x4 col;
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) col[kk] = kk < ka ? ra[r][kk][i] : 0.f;
                     bf16x4_t hi, lo;
                     split4(col, hi, lo);
                     const int off = (4 * (a_m4 + r * T::A_MSTEP) + i) * T::RS + 8 * a_k4;
                     *reinterpret_cast<bf16x4_t*>(Ahi + off) = hi;
                     *reinterpret_cast<bf16x4_t*>(Alo + off) = lo;
                 }
+            const int kb = K - (k0 + b_k);
 #pragma unroll
             for (int r = 0; r < T::B_VEC; ++r) {
+                f32x4 v;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[e] = e < kb ? rb[r][e] : 0.f;
                 bf16x4_t hi, lo;
-                split4(rb[r], hi, lo);
+                split4(v, hi, lo);
                 const int off = (b_n + r * T::B_NSTEP) * T::RS + 2 * b_k;
                 *reinterpret_cast<bf16x4_t*>(Bhi + off) = hi;
                 *reinterpret_cast<bf16x4_t*>(Blo + off) = lo;
@@ -179,7 +179,7 @@ struct X3TileRunner {
         const int ktiles = div_up(K, BK);
         if (ktiles > 0) {
             gload(0);
-            sstore(smem);
+            sstore(smem, 0);
             __syncthreads();
         }
         for (int kt = 0; kt < ktiles; ++kt) {
@@ -230,7 +230,7 @@ struct X3TileRunner {
                         }
                     }
             }
-            if (more) sstore(smem + ((kt + 1) & 1) * T::STAGE);
+            if (more) sstore(smem + ((kt + 1) & 1) * T::STAGE, (kt + 1) * BK);
             __syncthreads();
         }
 

@@ -68,6 +68,13 @@ def test_config_queries(lib):
     assert "gfx950" in sa.version()
 
 
+def test_huge_leading_dimension_not_supported(lib):
+    p = 4096
+    big = (1 << 21) + 1
+    assert sa.sgemm_ex(4, 4, 4, 1.0, p, big, p, 4, 0.0, p, 4, check=False) == 2
+    assert sa.sgemm_ex(4, 4, 4, 1.0, p, 4, p, big, 0.0, p, 4, check=False) == 2
+
+
 def test_unknown_config_rejected(lib):
     p = 4096
     assert sa.run_config(0, 999, 4, 4, 4, 1.0, p, 4, p, 4, 0.0, p, 4, check=False) == 2
