@@ -113,6 +113,16 @@ def lib() -> ctypes.CDLL:
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"libsgemm_amd.so not found at {LIB_PATH}: build it first "
                            f"(make -C {ROOT})")
+    # One HIP runtime per process.  torch's ROCm build ships its own
+    # libamdhip64 and asks for it by the unversioned name, so if this library
+    # loaded /opt/rocm's libamdhip64.so.7 first, importing torch afterwards
+    # would bring a second HIP/HSA runtime into the process (no device seen by
+    # one of them, crashes in the other).  Loading torch first makes this
+    # library bind to torch's runtime through its SONAME instead.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     i, f, vp = ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_void_p
     mm = [i, i, i, f, vp, i, vp, i, f, vp, i]
