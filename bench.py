@@ -39,8 +39,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-bf16x3", action="store_true")
     ap.add_argument("--no-error", action="store_true", help="skip the FP64 error report")
-    ap.add_argument("--cpu-cols", type=int, default=48,
-                    help="output columns of the CPU baseline sample")
+    ap.add_argument("--cpu-cols", type=int, default=0,
+                    help="output columns of the CPU baseline sample (0 = all: the whole problem)")
     ap.add_argument("--no-grouped", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--group-count", type=int, default=512)
@@ -105,6 +105,7 @@ def cpu_baseline(size: int, cols: int):
     import numpy as np
     import oracle as orc
     m, k = size, size
+    cols = cols if cols > 0 else size
     A = orc.fill("uniform", m * k, 1)
     B = orc.fill("uniform", k * cols, 2)
     C = orc.fill("uniform", m * cols, 3)
@@ -114,8 +115,8 @@ def cpu_baseline(size: int, cols: int):
     flops = 2.0 * m * cols * k
     return {"value": flops / dt / 1e12, "unit": "TFLOPS", "cores": orc.max_threads(),
             "kind": "port",
-            "sample": f"{cols} output columns of the {size}^3 problem (M={m}, N={cols}, K={k}), "
-                      f"alpha=1.5 beta=0.5, {dt:.1f} s"}
+            "sample": f"{cols} of {size} output columns of the {size}^3 problem "
+                      f"(M={m}, N={cols}, K={k}), alpha=1.5 beta=0.5, {dt:.1f} s"}
 
 
 def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0):

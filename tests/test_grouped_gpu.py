@@ -27,8 +27,8 @@ def _log_uniform_shapes(count, lo=32, hi=2048, seed=0):
 def _make_group(shapes, seed=0, kind="uniform", alpha=1.0, beta=0.0, ld_pad=0):
     cases, dev, probs = [], [], []
     for i, (m, n, k) in enumerate(shapes):
-        c = Case(m, n, k, lda=m + ld_pad, ldb=k + ld_pad, ldc=m + ld_pad, kind=kind,
-                 seed=seed + 17 * i, alpha=alpha, beta=beta)
+        c = Case(m, n, k, lda=max(1, m + ld_pad), ldb=max(1, k + ld_pad),
+                 ldc=max(1, m + ld_pad), kind=kind, seed=seed + 17 * i, alpha=alpha, beta=beta)
         dA, dB, dC = c.device()
         cases.append(c)
         dev.append((dA, dB, dC))
