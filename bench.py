@@ -46,15 +46,19 @@ def parse():
     ap.add_argument("--group-count", type=int, default=512)
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="arrival window per swept rate")
-    ap.add_argument("--serve-rates", type=str, default="0.5,0.9",
+    ap.add_argument("--serve-rates", type=str, default="0.25,0.5,0.75,0.9",
                     help="arrival rates as fractions of the measured closed-loop capacity")
     ap.add_argument("--plain-only", action="store_true",
                     help="time only the plain FP32 kernel (profiling runs)")
+    ap.add_argument("--x3-only", action="store_true",
+                    help="time only the BF16x3 kernel (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.plain_only:
         args.no_bf16x3 = args.no_error = args.no_cpu = args.no_grouped = args.no_serve = True
+    if args.x3_only:
+        args.no_error = args.no_cpu = args.no_grouped = args.no_serve = True
     return args
 
 
@@ -285,7 +289,8 @@ def main():
 
     print(f"bench: plain FP32 {n}^3, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
           flush=True)
-    ms = time_form(plain, args.steps, args.warmup, stream)
+    ms = time_form(plain, 1 if args.x3_only else args.steps, 0 if args.x3_only else args.warmup,
+                   stream)
     tflops = flops / (ms * 1e-3) / 1e12
     forms = {"f32": {"tflops": tflops, "ms": ms}}
 

@@ -83,6 +83,10 @@ int heuristic(int form, int grouped, int m, int n) {
         return t64 >= 512 ? 2 : 1;
     }
     if (grouped) return (m >= 96 && n >= 96) ? 0 : 1;
+    // 256x256 tiles with 8 waves (config 2) run fastest from ~3k^2 up; below
+    // that they leave CUs idle.
+    const long long t256 = (long long)div_up(m, 256) * div_up(n, 256);
+    if (t256 >= 128) return 2;
     const long long t128 = (long long)div_up(m, 128) * div_up(n, 128);
     return t128 >= 256 ? 0 : 1;
 }
