@@ -52,6 +52,10 @@ def parse():
                     help="time only the plain FP32 kernel (profiling runs)")
     ap.add_argument("--x3-only", action="store_true",
                     help="time only the BF16x3 kernel (profiling runs)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="size sweep instead of the single-size line: square 1024..16384 step "
+                         "1024 plus (1000,1000,1000), (4097,3071,2049), (8192,8192,127); "
+                         "alpha=1.5, beta=0.5; both forms")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -60,6 +64,38 @@ def parse():
     if args.x3_only:
         args.no_error = args.no_cpu = args.no_grouped = args.no_serve = True
     return args
+
+
+def sweep(sa, torch, steps, warmup):
+    """TFLOPS and fraction of peak per size for both forms (one JSON line)."""
+    shapes = [(s, s, s) for s in range(1024, 16385, 1024)]
+    shapes += [(1000, 1000, 1000), (4097, 3071, 2049), (8192, 8192, 127)]
+    stream = torch.cuda.current_stream()
+    out = []
+    for (m, n, k) in shapes:
+        A = torch.rand(m * k, device="cuda") * 2 - 1
+        B = torch.rand(k * n, device="cuda") * 2 - 1
+        C = torch.rand(m * n, device="cuda") * 2 - 1
+        flops = 2.0 * m * n * k
+        it = max(3, min(steps, int(2e13 / flops)))
+        row = {"m": m, "n": n, "k": k}
+        for form, name, fn, peak in ((sa.FORM_F32, "f32", sa.sgemm_ex, PEAK_FP32_TFLOPS),
+                                     (sa.FORM_BF16X3, "bf16x3", sa.sgemm_bf16x3,
+                                      PEAK_BF16_TFLOPS / 3)):
+            ms = time_form(lambda: fn(m, n, k, 1.5, A, m, B, k, 0.5, C, m, stream), it, warmup,
+                           stream)
+            tf = flops / (ms * 1e-3) / 1e12
+            cfg = sa.autotune_lookup(form, m, n, k)[0]
+            row[name] = {"ms": ms, "tflops": tf, "frac_of_peak": tf / peak, "config": cfg}
+        print(f"sweep {m}x{n}x{k}: f32 {row['f32']['tflops']:.1f} TF, "
+              f"bf16x3 {row['bf16x3']['tflops']:.1f} TF", file=sys.stderr, flush=True)
+        out.append(row)
+        del A, B, C
+        torch.cuda.empty_cache()
+    print(json.dumps({"metric": "sgemm size sweep", "unit": "TFLOPS",
+                      "peaks": {"f32": PEAK_FP32_TFLOPS, "bf16x3_useful": PEAK_BF16_TFLOPS / 3},
+                      "alpha": 1.5, "beta": 0.5, "data": "synthetic uniform [-1,1)",
+                      "rows": out}), flush=True)
 
 
 def copy_calibration(steps):
@@ -271,6 +307,10 @@ def main():
         sys.exit(1)
     if args.copy_calibration:
         copy_calibration(args.steps)
+        return
+    if args.sweep:
+        sa.lib()
+        sweep(sa, torch, args.steps, args.warmup)
         return
     sa.lib()
     torch.manual_seed(0)

@@ -85,8 +85,14 @@ struct F32TileRunner {
         // stored; k >= K is zeroed explicitly on both operands.
         const int a_m = 4 * (t % (BM / 4));  // row inside the tile
         const int a_k = t / (BM / 4);        // first k row inside the tile
-        const int b_k = 4 * (t % (BK / 4));
-        const int b_n = t / (BK / 4);
+        // B: float4 along k; BK/4 lanes cover one column.  Each 16-lane
+        // group (one 256-byte LDS pass) writes rows spaced BK/4 apart, which
+        // the BK+4 row stride puts on disjoint banks.
+        constexpr int S = BK / 4, R = 16 / S;
+        static_assert(16 % S == 0, "B store mapping");
+        const int b_k = 4 * (t % S);
+        const int b_q = t / 16;
+        const int b_n = (b_q % S) + S * ((t / S) % R) + 16 * (b_q / S);
         int a_off[T::A_VEC], b_off[T::B_VEC];
 #pragma unroll
         for (int r = 0; r < T::A_VEC; ++r) a_off[r] = 4 * (a_m + (a_k + r * T::A_KSTEP) * P.lda);
@@ -166,8 +172,11 @@ struct F32TileRunner {
                         for (int j = 0; j < T::NB; ++j)
                             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][s], b[j][s],
                                                                              acc[i][j], 0, 0, 0);
+                // Stage the next k-tile behind the first k-step's MFMAs, so
+                // the wait on its loads, the masking and the LDS stores
+                // overlap matrix-core work instead of following all of it.
+                if (kk == 0 && more) sstore(smem + ((kt + 1) & 1) * T::STAGE, (kt + 1) * BK);
             }
-            if (more) sstore(smem + ((kt + 1) & 1) * T::STAGE, (kt + 1) * BK);
             __syncthreads();
         }
 

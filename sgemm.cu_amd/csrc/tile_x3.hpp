@@ -102,8 +102,14 @@ struct X3TileRunner {
         // never reach a stored C element, k >= K is zeroed explicitly.
         const int a_k4 = t % (BK / 4);
         const int a_m4 = t / (BK / 4);
-        const int b_k = 4 * (t % (BK / 4));
-        const int b_n = t / (BK / 4);
+        // B: float4 along k, BK/4 lanes per column; each 32-lane group (one
+        // 256-byte pass of 8-byte LDS stores) writes rows spaced so that the
+        // 2*BK+16-byte row stride puts them on disjoint banks.
+        constexpr int S = BK / 4, R = 32 / S, SP = 16 / R;
+        static_assert(32 % S == 0 && 16 % R == 0, "B store mapping");
+        const int b_k = 4 * (t % S);
+        const int b_q = t / 32;
+        const int b_n = (b_q % SP) + SP * ((t / S) % R) + 16 * (b_q / SP);
         int a_off[T::A_SUB], b_off[T::B_VEC];
 #pragma unroll
         for (int r = 0; r < T::A_SUB; ++r)
@@ -229,8 +235,10 @@ struct X3TileRunner {
                             acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
                         }
                     }
+                // Stage the next k-tile (wait on its loads, split, LDS stores)
+                // behind the first k-step's MFMAs.
+                if (kk == 0 && more) sstore(smem + ((kt + 1) & 1) * T::STAGE, (kt + 1) * BK);
             }
-            if (more) sstore(smem + ((kt + 1) & 1) * T::STAGE, (kt + 1) * BK);
             __syncthreads();
         }
 
