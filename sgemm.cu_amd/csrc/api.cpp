@@ -10,7 +10,7 @@
 #include <cstdio>
 #include <cstring>
 
-#include "../../include/sgemm.h"
+#include "sgemm.h"
 #include "autotune.hpp"
 #include "kernels.hpp"
 

@@ -19,7 +19,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/sgemm.h"
+#include "sgemm.h"
 #include "autotune.hpp"
 #include "kernels.hpp"
 

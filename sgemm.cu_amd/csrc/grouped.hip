@@ -29,7 +29,7 @@
 #include <numeric>
 #include <vector>
 
-#include "../../include/sgemm.h"
+#include "sgemm.h"
 #include "autotune.hpp"
 #include "grouped.hpp"
 #include "tile_f32.hpp"

@@ -3,7 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
-#include "../../include/sgemm.h"
+#include "sgemm.h"
 #include "kernels.hpp"
 
 namespace sgemm_amd {

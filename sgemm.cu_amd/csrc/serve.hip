@@ -39,7 +39,7 @@
 #include <unordered_map>
 #include <vector>
 
-#include "../../include/sgemm.h"
+#include "sgemm.h"
 #include "grouped.hpp"
 
 namespace sgemm_amd {

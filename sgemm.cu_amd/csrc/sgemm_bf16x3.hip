@@ -6,8 +6,10 @@ namespace sgemm_amd {
 
 constexpr int kX3GroupN = 8;
 
+// WPE: waves per SIMD the register allocation must allow (T::WPE).
 template <class T, bool VEC>
-__global__ __launch_bounds__(T::NT) void k_sgemm_bf16x3(Problem P) {
+__global__ __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(T::WPE)))
+void k_sgemm_bf16x3(Problem P) {
     __shared__ __attribute__((aligned(16))) char smem[T::LDS_BYTES];
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     int tm, tn;
@@ -21,6 +23,7 @@ static const ConfigInfo kBf16x3Configs[] = {
     {256, 256, 32, 2, 4, "bf16x3_256x256x32_w2x4_m32_single"},
     {256, 128, 32, 2, 2, "bf16x3_256x128x32_w2x2_m32_single"},
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m32_dual"},
+    {256, 128, 16, 2, 2, "bf16x3_256x128x16_w2x2_m32_single_mid"},
 };
 
 int bf16x3_config_count() { return (int)(sizeof(kBf16x3Configs) / sizeof(kBf16x3Configs[0])); }

@@ -36,12 +36,20 @@ typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 // 32 -> v_mfma_f32_32x32x16_bf16 (32x32 blocks, 16 accumulators per lane).
 // DUAL keeps hi*lo + lo*hi in separate accumulators (added in the epilogue);
 // otherwise all three products go into one set, which halves the accumulator
-// registers and so allows larger wave tiles.
-template <int BM_, int BN_, int BK_, int WM_, int WN_, int MF_ = 16, bool DUAL_ = true>
+// registers and so allows larger wave tiles.  MID stages the next k-tile
+// right after the first k-step's MFMAs instead of after the last one; it pays
+// when two independent workgroups share a CU and hurts a lone wave per SIMD,
+// whose loads then have less time to land.
+template <int BM_, int BN_, int BK_, int WM_, int WN_, int MF_ = 16, bool DUAL_ = true,
+          bool MID_ = false>
 struct X3Tile {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
     static constexpr int MF = MF_;
     static constexpr bool DUAL = DUAL_;
+    static constexpr bool MID = MID_;
+    // Waves per SIMD the kernel's registers must allow: two independent
+    // workgroups per CU for the MID configs, else whatever the tile needs.
+    static constexpr int WPE = MID_ ? 2 : 1;
     static constexpr int NT = 64 * WM * WN;
     static constexpr int TM = BM / WM, TN = BN / WN;
     static constexpr int MB = TM / MF, NB = TN / MF;     // MFMA blocks per wave
@@ -56,7 +64,7 @@ struct X3Tile {
     static constexpr int B_VEC = BN * BK / 4 / NT;          // float4 of B per thread
     static constexpr int B_NSTEP = NT / (BK / 4);
     static_assert(MF == 16 || MF == 32, "MFMA shape");
-    static_assert(BK % 32 == 0, "BK must be a multiple of 32");
+    static_assert(BK % KSTEP == 0 && BK % 16 == 0, "BK must be a multiple of the MFMA k");
     static_assert(TM % MF == 0 && TN % MF == 0, "wave tile");
     static_assert(NT % (BK / 4) == 0, "load mapping");
     static_assert(A_SUB >= 1 && (BM / 4) * (BK / 4) % NT == 0, "A split");
@@ -236,8 +244,9 @@ struct X3TileRunner {
                         }
                     }
                 // Stage the next k-tile (wait on its loads, split, LDS stores)
-                // behind the first k-step's MFMAs.
-                if (kk == 0 && more) sstore(smem + ((kt + 1) & 1) * T::STAGE, (kt + 1) * BK);
+                // behind the first k-step's MFMAs, or after the last one.
+                constexpr int kStoreAt = T::MID ? 0 : BK - T::KSTEP;
+                if (kk == kStoreAt && more) sstore(smem + ((kt + 1) & 1) * T::STAGE, (kt + 1) * BK);
             }
             __syncthreads();
         }
@@ -292,6 +301,7 @@ using X3_64x64 = X3Tile<64, 64, 64, 2, 2>;
 using X3_256x256_w8 = X3Tile<256, 256, 32, 2, 4, 32, false>;
 using X3_256x128_w4 = X3Tile<256, 128, 32, 2, 2, 32, false>;
 using X3_128x128_m32 = X3Tile<128, 128, 32, 2, 2, 32, true>;
+using X3_256x128_k16 = X3Tile<256, 128, 16, 2, 2, 32, false, true>;
 
 template <class F>
 __host__ __device__ inline bool dispatch_x3(int id, F&& f) {
@@ -301,6 +311,7 @@ __host__ __device__ inline bool dispatch_x3(int id, F&& f) {
         case 2: f(TypeTag<X3_256x256_w8>{}); return true;
         case 3: f(TypeTag<X3_256x128_w4>{}); return true;
         case 4: f(TypeTag<X3_128x128_m32>{}); return true;
+        case 5: f(TypeTag<X3_256x128_k16>{}); return true;
         default: return false;
     }
 }

@@ -25,7 +25,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/sgemm.h"
+#include "sgemm.h"
 #include "harness_util.hpp"
 
 using namespace harness;

@@ -83,6 +83,14 @@ def test_config1_1024(alpha, beta):
     assert_ok(st, f"1024^3 alpha={alpha} beta={beta}")
 
 
+@pytest.mark.parametrize("n", [3072, 5120, 7168, 9216, 12288])
+def test_square_sweep_sampled_fp64(n):
+    """More squares of the 1024..16384 sweep: sampled entries against FP64."""
+    c = Case(n, n, n, seed=n)
+    out = run_form(c, sa.FORM_F32)
+    _sampled_check(c, out, n_samples=2048, seed=n)
+
+
 @pytest.mark.parametrize("config", range(N_F32_CONFIGS))
 def test_integer_kat_bit_exact(config):
     """Integer inputs in [-4, 4]: every partial sum is exact, so any summation
@@ -218,7 +226,7 @@ def _sampled_check(c: Case, out: np.ndarray, n_samples=4096, seed=0):
 
 
 @pytest.mark.parametrize("m,n,k", [(1000, 1000, 1000), (2048, 2048, 2048), (4097, 3071, 2049),
-                                   (8192, 8192, 127), (8192, 8192, 8192)])
+                                   (8192, 8192, 127), (8192, 8192, 8192), (16384, 16384, 16384)])
 def test_sweep_shapes_sampled_fp64_and_full_vs_naive(m, n, k):
     """Baseline sweep shapes: sampled FP64 entries, plus every entry against the
     independent naive kernel (itself bit-exact to the oracle's restatement)."""
