@@ -8,7 +8,8 @@ namespace sgemm_amd {
 constexpr int kGroupN = 8;
 
 template <class T, bool VEC>
-__global__ __launch_bounds__(T::NT) void k_sgemm_f32(Problem P) {
+__global__ __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(T::WPE)))
+void k_sgemm_f32(Problem P) {
     __shared__ __attribute__((aligned(16))) float smem[T::LDS_FLOATS];
     const int id = xcd_remap(blockIdx.x, gridDim.x);
     int tm, tn;
@@ -21,6 +22,8 @@ static const ConfigInfo kF32Configs[] = {
     {64, 64, 32, 2, 2, "f32_64x64x32_w2x2"},
     {128, 64, 32, 2, 2, "f32_128x64x32_w2x2"},
     {256, 128, 16, 2, 2, "f32_256x128x16_w2x2"},
+    {128, 128, 32, 2, 2, "f32_128x128x32_w2x2_akmajor"},
+    {128, 128, 16, 2, 2, "f32_128x128x16_w2x2_3wpe"},
 };
 
 int f32_config_count() { return (int)(sizeof(kF32Configs) / sizeof(kF32Configs[0])); }

@@ -30,15 +30,24 @@
 
 namespace sgemm_amd {
 
-template <int BM_, int BN_, int BK_, int WM_, int WN_>
+// AT_: keep A k-contiguous in LDS (As[m][k], like B) instead of m-contiguous
+// (As[k][m], as in HBM).  Each thread then loads a 4(m) x 4(k) sub-block of A
+// and transposes it in registers; in exchange every A fragment is one
+// ds_read_b128 (four MFMA steps) instead of four 4-byte reads.
+// WPE_: waves per SIMD the register allocation must allow.
+template <int BM_, int BN_, int BK_, int WM_, int WN_, bool AT_ = false, int WPE_ = 1>
 struct F32Tile {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
+    static constexpr bool AT = AT_;
+    static constexpr int WPE = WPE_;
     static constexpr int NT = 64 * WM * WN;
     static constexpr int TM = BM / WM, TN = BN / WN;  // wave tile
     static constexpr int MB = TM / 16, NB = TN / 16;  // 16x16 MFMA blocks per wave
-    static constexpr int SA = BM + 4;                 // As[k][m] row stride (floats)
+    // A layout: As[k][m] with row stride BM+4, or As[m][k] with row stride BK+4.
+    static constexpr int SA = AT ? BK + 4 : BM + 4;
+    static constexpr int A_FLOATS = AT ? BM * SA : BK * SA;
     static constexpr int SB = BK + 4;                 // Bs[n][k] row stride (floats)
-    static constexpr int STAGE = BK * SA + BN * SB;   // floats per LDS stage
+    static constexpr int STAGE = A_FLOATS + BN * SB;  // floats per LDS stage
     static constexpr int LDS_FLOATS = 2 * STAGE;
     static constexpr int A_VEC = BM * BK / 4 / NT;  // float4 of A per thread per k-tile
     static constexpr int B_VEC = BN * BK / 4 / NT;
@@ -48,6 +57,8 @@ struct F32Tile {
     static_assert(BK % 16 == 0, "BK must be a multiple of 16");
     static_assert((BM * BK / 4) % NT == 0 && (BN * BK / 4) % NT == 0, "load split");
     static_assert(NT % (BM / 4) == 0 && NT % (BK / 4) == 0, "load mapping");
+    static_assert(!AT || (A_VEC % 4 == 0 && (BM / 4) * (BK / 4) == NT * (A_VEC / 4)),
+                  "transposed A needs whole 4x4 sub-blocks per thread");
     static_assert(LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
 };
 
@@ -83,8 +94,15 @@ struct F32TileRunner {
         // columns >= N read either 0 (past the operand's extent) or
         // in-extent values that only reach C rows/columns that are never
         // stored; k >= K is zeroed explicitly on both operands.
+        // A, m-contiguous layout: float4 along m at (a_m, a_k + r*A_KSTEP).
         const int a_m = 4 * (t % (BM / 4));  // row inside the tile
         const int a_k = t / (BM / 4);        // first k row inside the tile
+        // A, k-contiguous layout: 4x4 sub-block at rows 4*at_m4.., k 4*at_k4..;
+        // a 16-lane group stores two rows 8 apart (disjoint banks at BK+4).
+        constexpr int AS = BK / 4;
+        const int at_k4 = t % AS;
+        const int at_q = t / 16;
+        const int at_m4 = (at_q % 2) + 2 * ((t / AS) % 2) + 4 * (at_q / 2);
         // B: float4 along k; BK/4 lanes cover one column.  Each 16-lane
         // group (one 256-byte LDS pass) writes rows spaced BK/4 apart, which
         // the BK+4 row stride puts on disjoint banks.
@@ -95,7 +113,13 @@ struct F32TileRunner {
         const int b_n = (b_q % S) + S * ((t / S) % R) + 16 * (b_q / S);
         int a_off[T::A_VEC], b_off[T::B_VEC];
 #pragma unroll
-        for (int r = 0; r < T::A_VEC; ++r) a_off[r] = 4 * (a_m + (a_k + r * T::A_KSTEP) * P.lda);
+        for (int r = 0; r < T::A_VEC; ++r) {
+            if constexpr (T::AT)
+                a_off[r] = 4 * (4 * (at_m4 + (r / 4) * (T::NT / AS)) +
+                                (4 * at_k4 + r % 4) * P.lda);
+            else
+                a_off[r] = 4 * (a_m + (a_k + r * T::A_KSTEP) * P.lda);
+        }
 #pragma unroll
         for (int r = 0; r < T::B_VEC; ++r) b_off[r] = 4 * (b_k + (b_n + r * T::B_NSTEP) * P.ldb);
         const long long a_ext = extent(M, K, P.lda), b_ext = extent(K, N, P.ldb);
@@ -116,11 +140,26 @@ struct F32TileRunner {
         };
         auto sstore = [&](float* stage, int k0) {
             float* As = stage;
-            float* Bs = stage + BK * T::SA;
+            float* Bs = stage + T::A_FLOATS;
+            if constexpr (T::AT) {
+                const int ka = K - (k0 + 4 * at_k4);  // valid k in this sub-block
 #pragma unroll
-            for (int r = 0; r < T::A_VEC; ++r) {
-                const f32x4 v = (k0 + a_k + r * T::A_KSTEP < K) ? ra[r] : f32x4{0.f, 0.f, 0.f, 0.f};
-                *reinterpret_cast<f32x4*>(&As[(a_k + r * T::A_KSTEP) * T::SA + a_m]) = v;
+                for (int sb = 0; sb < T::A_VEC / 4; ++sb)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        f32x4 v;  // row 4*m4 + i, k = 4*at_k4 .. +3
+#pragma unroll
+                        for (int kk = 0; kk < 4; ++kk) v[kk] = kk < ka ? ra[4 * sb + kk][i] : 0.f;
+                        const int row = 4 * (at_m4 + sb * (T::NT / AS)) + i;
+                        *reinterpret_cast<f32x4*>(&As[row * T::SA + 4 * at_k4]) = v;
+                    }
+            } else {
+#pragma unroll
+                for (int r = 0; r < T::A_VEC; ++r) {
+                    const f32x4 v =
+                        (k0 + a_k + r * T::A_KSTEP < K) ? ra[r] : f32x4{0.f, 0.f, 0.f, 0.f};
+                    *reinterpret_cast<f32x4*>(&As[(a_k + r * T::A_KSTEP) * T::SA + a_m]) = v;
+                }
             }
             const int kv = K - (k0 + b_k);  // valid k in each B float4
 #pragma unroll
@@ -146,20 +185,26 @@ struct F32TileRunner {
         }
         for (int kt = 0; kt < ktiles; ++kt) {
             const float* As = smem + (kt & 1) * T::STAGE;
-            const float* Bs = As + BK * T::SA;
+            const float* Bs = As + T::A_FLOATS;
             const bool more = kt + 1 < ktiles;
             if (more) gload((kt + 1) * BK);
 #pragma unroll
             for (int kk = 0; kk < BK; kk += 16) {
                 // Lane group g supplies k = kk + 4g + s at MFMA step s, for A
                 // and B alike, so B's four steps are one 16-byte LDS read.
-                float a[T::MB][4];
+                f32x4 a[T::MB];
                 f32x4 b[T::NB];
 #pragma unroll
-                for (int i = 0; i < T::MB; ++i)
+                for (int i = 0; i < T::MB; ++i) {
+                    if constexpr (T::AT) {
+                        a[i] = *reinterpret_cast<const f32x4*>(
+                            &As[(wm * T::TM + 16 * i + l16) * T::SA + kk + 4 * g]);
+                    } else {
 #pragma unroll
-                    for (int s = 0; s < 4; ++s)
-                        a[i][s] = As[(kk + 4 * g + s) * T::SA + wm * T::TM + 16 * i + l16];
+                        for (int s = 0; s < 4; ++s)
+                            a[i][s] = As[(kk + 4 * g + s) * T::SA + wm * T::TM + 16 * i + l16];
+                    }
+                }
 #pragma unroll
                 for (int j = 0; j < T::NB; ++j)
                     b[j] = *reinterpret_cast<const f32x4*>(
@@ -219,6 +264,8 @@ using F32_128x128 = F32Tile<128, 128, 32, 2, 2>;
 using F32_64x64 = F32Tile<64, 64, 32, 2, 2>;
 using F32_128x64 = F32Tile<128, 64, 32, 2, 2>;
 using F32_256x128 = F32Tile<256, 128, 16, 2, 2>;
+using F32_128x128_at = F32Tile<128, 128, 32, 2, 2, true, 2>;
+using F32_128x128_k16 = F32Tile<128, 128, 16, 2, 2, false, 3>;
 
 template <class T>
 struct TypeTag {
@@ -233,6 +280,8 @@ __host__ __device__ inline bool dispatch_f32(int id, F&& f) {
         case 1: f(TypeTag<F32_64x64>{}); return true;
         case 2: f(TypeTag<F32_128x64>{}); return true;
         case 3: f(TypeTag<F32_256x128>{}); return true;
+        case 4: f(TypeTag<F32_128x128_at>{}); return true;
+        case 5: f(TypeTag<F32_128x128_k16>{}); return true;
         default: return false;
     }
 }

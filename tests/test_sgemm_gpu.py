@@ -23,7 +23,7 @@ from gpu_util import Case, assert_ok, check, compact, download, log, run_form, u
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gemm_cases.json")
-N_F32_CONFIGS = 4
+N_F32_CONFIGS = 6
 
 
 @pytest.fixture(scope="module")
