@@ -32,6 +32,53 @@ struct Problem {
 
 constexpr int kProblemVec4 = 1;
 
+// Split-K state of one workgroup: it computes k in [split*kchunk, ...) of
+// tile `tile`, dumps its accumulators to `partials`, and the last of the
+// `splits` workgroups of that tile to arrive (counted in counters[tile]) sums
+// all partials in split order 0, 1, ..., splits-1 -- so the result does not
+// depend on which workgroup finishes last -- and runs the epilogue.
+//
+// Coherence across XCDs (each has its own L2): the partials are written and
+// read with the sc1 cache policy, the policy of agent-scope atomic loads and
+// stores, so they pass through to the device-wide coherence point instead of
+// sitting in one XCD's L2.  The writer waits for its stores to complete before
+// the counter increment, which is then all the release the reader needs; no
+// agent-scope fence (a whole-L2 writeback plus invalidate) is issued.
+struct SplitK {
+    float* partials;  // [tiles][splits][accumulators][threads] float4s
+    int* counters;    // [tiles], zero between launches
+    int splits;
+    int kchunk;       // multiple of the tile's BK
+};
+
+// Cache policy of the partials' loads and stores: sc1 (agent coherence).
+constexpr int kPolicyAgent = 0x10;
+
+__device__ inline void store_agent(__amdgpu_buffer_rsrc_t r, int off, f32x4 v) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off, 0, kPolicyAgent);
+}
+__device__ inline f32x4 load_agent(__amdgpu_buffer_rsrc_t r, int off) {
+    return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, kPolicyAgent));
+}
+
+// Called by every thread of the workgroup after its partial stores.  Returns
+// true in the workgroup that arrives last for `tile` (and resets the counter
+// for the next launch).
+__device__ inline bool splitk_arrive(const SplitK& sk, int tile, int* flag_lds) {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);  // keep the stores above the wait
+    __builtin_amdgcn_s_waitcnt(0);            // this wave's partial stores completed
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();                          // ... and every other wave's
+    if (threadIdx.x == 0)
+        *flag_lds = __hip_atomic_fetch_add(&sk.counters[tile], 1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const bool last = *flag_lds == sk.splits - 1;
+    if (last && threadIdx.x == 0)
+        __hip_atomic_store(&sk.counters[tile], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return last;
+}
+
 static_assert(sizeof(Problem) % 8 == 0, "descriptor packing");
 
 __host__ __device__ constexpr int div_up(int a, int b) { return (a + b - 1) / b; }

@@ -12,7 +12,17 @@ struct ConfigInfo {
     int bm, bn, bk;  // workgroup tile
     int wm, wn;      // waves along m and n
     const char* name;
+    int splitk;      // 1: the k range is split across workgroups (splitk_count)
 };
+
+// Number of k-splits for a tiles-sized grid: enough workgroups for two per
+// CU, each split keeping at least four k-tiles, at most 16 splits.
+int splitk_count(long long tiles, int k, int bk);
+
+// Per-stream split-K workspace (partials + zeroed per-tile counters), grown on
+// demand; the first call on a stream, or a larger problem, allocates.
+hipError_t splitk_workspace(hipStream_t stream, size_t partial_bytes, int tiles,
+                            float** partials, int** counters);
 
 // Plain FP32 configs (sgemm_f32.hip).
 int f32_config_count();

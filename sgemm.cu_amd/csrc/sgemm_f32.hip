@@ -7,16 +7,21 @@ namespace sgemm_amd {
 // Column bands of this many tile-columns in the launch order.
 constexpr int kGroupN = 8;
 
-template <class T, bool VEC>
+// SPLIT: the grid is tiles x splits; workgroup id -> (split, tile) with the
+// split index outermost, so workgroups running together share k-chunks.
+template <class T, bool VEC, bool SPLIT>
 __global__ __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(T::WPE)))
-void k_sgemm_f32(Problem P) {
+void k_sgemm_f32(Problem P, SplitK sk) {
     __shared__ __attribute__((aligned(16))) float smem[T::LDS_FLOATS];
     const int id = xcd_remap(blockIdx.x, gridDim.x);
+    const int tiles = P.tiles_m * P.tiles_n;
+    const int split = SPLIT ? id / tiles : 0;
     int tm, tn;
-    tile_coords(id, P.tiles_m, P.tiles_n, kGroupN, tm, tn);
-    F32TileRunner<T, VEC>::run(P, tm, tn, smem);
+    tile_coords(SPLIT ? id - split * tiles : id, P.tiles_m, P.tiles_n, kGroupN, tm, tn);
+    F32TileRunner<T, VEC>::run(P, tm, tn, smem, SPLIT ? &sk : nullptr, split);
 }
 
+// Ids 6-8 run the tiles of ids 0-2 with split-K (splitk = 1).
 static const ConfigInfo kF32Configs[] = {
     {128, 128, 32, 2, 2, "f32_128x128x32_w2x2"},
     {64, 64, 32, 2, 2, "f32_64x64x32_w2x2"},
@@ -24,12 +29,15 @@ static const ConfigInfo kF32Configs[] = {
     {256, 128, 16, 2, 2, "f32_256x128x16_w2x2"},
     {128, 128, 32, 2, 2, "f32_128x128x32_w2x2_akmajor"},
     {128, 128, 16, 2, 2, "f32_128x128x16_w2x2_3wpe"},
+    {128, 128, 32, 2, 2, "f32_128x128x32_w2x2_splitk", 1},
+    {64, 64, 32, 2, 2, "f32_64x64x32_w2x2_splitk", 1},
+    {128, 64, 32, 2, 2, "f32_128x64x32_w2x2_splitk", 1},
 };
 
 int f32_config_count() { return (int)(sizeof(kF32Configs) / sizeof(kF32Configs[0])); }
 const ConfigInfo& f32_config(int id) { return kF32Configs[id]; }
 
-template <class T>
+template <class T, bool ALLOW_SPLIT>
 static hipError_t launch_tile(const Problem& p, hipStream_t s) {
     Problem q = p;
     q.tiles_m = div_up(p.m, T::BM);
@@ -37,17 +45,41 @@ static hipError_t launch_tile(const Problem& p, hipStream_t s) {
     const long long tiles = (long long)q.tiles_m * q.tiles_n;
     if (tiles <= 0) return hipSuccess;
     if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
+    SplitK sk{nullptr, nullptr, 1, p.k};
+    if (ALLOW_SPLIT) {
+        sk.splits = splitk_count(tiles, p.k, T::BK);
+        sk.kchunk = div_up(div_up(p.k, sk.splits), T::BK) * T::BK;
+        sk.splits = div_up(p.k, sk.kchunk);  // every split gets k
+    }
+    const bool split = ALLOW_SPLIT && sk.splits > 1;
+    if (split) {
+        const size_t bytes = (size_t)tiles * sk.splits * T::MB * T::NB * T::NT * 16;
+        hipError_t e = splitk_workspace(s, bytes, (int)tiles, &sk.partials, &sk.counters);
+        if (e != hipSuccess) return e;
+    }
+    const dim3 grid((unsigned)(tiles * sk.splits));
     (void)hipGetLastError();  // the status read below must belong to this launch
-    if (vec4_ok(p))
-        hipLaunchKernelGGL((k_sgemm_f32<T, true>), dim3((unsigned)tiles), dim3(T::NT), 0, s, q);
-    else
-        hipLaunchKernelGGL((k_sgemm_f32<T, false>), dim3((unsigned)tiles), dim3(T::NT), 0, s, q);
+    constexpr bool S = ALLOW_SPLIT;
+    if (vec4_ok(p)) {
+        if (split) hipLaunchKernelGGL((k_sgemm_f32<T, true, S>), grid, dim3(T::NT), 0, s, q, sk);
+        else hipLaunchKernelGGL((k_sgemm_f32<T, true, false>), grid, dim3(T::NT), 0, s, q, sk);
+    } else {
+        if (split) hipLaunchKernelGGL((k_sgemm_f32<T, false, S>), grid, dim3(T::NT), 0, s, q, sk);
+        else hipLaunchKernelGGL((k_sgemm_f32<T, false, false>), grid, dim3(T::NT), 0, s, q, sk);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_f32(int id, const Problem& p, hipStream_t s) {
+    if (id < 0 || id >= f32_config_count()) return hipErrorInvalidValue;
+    switch (id) {
+        case 6: return launch_tile<F32_128x128, true>(p, s);
+        case 7: return launch_tile<F32_64x64, true>(p, s);
+        case 8: return launch_tile<F32_128x64, true>(p, s);
+        default: break;
+    }
     hipError_t e = hipErrorInvalidValue;
-    dispatch_f32(id, [&](auto tag) { e = launch_tile<typename decltype(tag)::type>(p, s); });
+    dispatch_f32(id, [&](auto tag) { e = launch_tile<typename decltype(tag)::type, false>(p, s); });
     return e;
 }
 

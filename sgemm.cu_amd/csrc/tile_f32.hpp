@@ -80,8 +80,10 @@ struct F32TileRunner {
     static constexpr int BM = T::BM, BN = T::BN, BK = T::BK;
 
     // Computes tile (tm, tn) of problem P.  `smem` holds T::LDS_FLOATS floats.
-    // All threads of the workgroup must call this together.
-    __device__ static void run(const Problem& P, int tm, int tn, float* smem) {
+    // All threads of the workgroup must call this together.  With `sk`, only
+    // split `split` of the k range runs here (see SplitK).
+    __device__ static void run(const Problem& P, int tm, int tn, float* smem,
+                               const SplitK* sk = nullptr, int split = 0) {
         const int t = threadIdx.x;
         const int lane = t & 63, wave = t >> 6;
         const int wm = wave % T::WM, wn = wave / T::WM;
@@ -177,17 +179,20 @@ struct F32TileRunner {
 #pragma unroll
             for (int j = 0; j < T::NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-        const int ktiles = div_up(K, BK);
+        const int kbeg = sk ? split * sk->kchunk : 0;
+        const int kend = sk ? min(K, kbeg + sk->kchunk) : K;
+        const int ktiles = kend > kbeg ? div_up(kend - kbeg, BK) : 0;
         if (ktiles > 0) {
-            gload(0);
-            sstore(smem, 0);
+            gload(kbeg);
+            sstore(smem, kbeg);
             __syncthreads();
         }
         for (int kt = 0; kt < ktiles; ++kt) {
             const float* As = smem + (kt & 1) * T::STAGE;
             const float* Bs = As + T::A_FLOATS;
             const bool more = kt + 1 < ktiles;
-            if (more) gload((kt + 1) * BK);
+            const int knext = kbeg + (kt + 1) * BK;
+            if (more) gload(knext);
 #pragma unroll
             for (int kk = 0; kk < BK; kk += 16) {
                 // Lane group g supplies k = kk + 4g + s at MFMA step s, for A
@@ -220,9 +225,55 @@ struct F32TileRunner {
                 // Stage the next k-tile behind the first k-step's MFMAs, so
                 // the wait on its loads, the masking and the LDS stores
                 // overlap matrix-core work instead of following all of it.
-                if (kk == 0 && more) sstore(smem + ((kt + 1) & 1) * T::STAGE, (kt + 1) * BK);
+                if (kk == 0 && more) sstore(smem + ((kt + 1) & 1) * T::STAGE, knext);
             }
             __syncthreads();
+        }
+
+        if (sk) {
+            // Dump the accumulators in lane order (coalesced 16-byte stores),
+            // then let the last workgroup of this tile reduce in split order;
+            // its own partial comes from its registers.
+            const int tile = tm * P.tiles_n + tn;
+            constexpr int NACC = T::MB * T::NB;
+            constexpr int SPLIT_BYTES = NACC * T::NT * 16;
+            const __amdgpu_buffer_rsrc_t part =
+                make_rsrc(sk->partials + (size_t)tile * sk->splits * NACC * T::NT * 4,
+                          (long long)sk->splits * SPLIT_BYTES);
+#pragma unroll
+            for (int i = 0; i < T::MB; ++i)
+#pragma unroll
+                for (int j = 0; j < T::NB; ++j)
+                    store_agent(part, split * SPLIT_BYTES + ((i * T::NB + j) * T::NT + t) * 16,
+                                acc[i][j]);
+            if (!splitk_arrive(*sk, tile, reinterpret_cast<int*>(smem))) return;
+            // One split's loads are all issued before any is waited on.
+            f32x4 sum[T::MB][T::NB];
+#pragma unroll
+            for (int i = 0; i < T::MB; ++i)
+#pragma unroll
+                for (int j = 0; j < T::NB; ++j)
+                    sum[i][j] = split == 0 ? acc[i][j]
+                                           : load_agent(part, ((i * T::NB + j) * T::NT + t) * 16);
+            for (int q = 1; q < sk->splits; ++q) {
+                if (q == split) {
+#pragma unroll
+                    for (int i = 0; i < T::MB; ++i)
+#pragma unroll
+                        for (int j = 0; j < T::NB; ++j) sum[i][j] += acc[i][j];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < T::MB; ++i)
+#pragma unroll
+                        for (int j = 0; j < T::NB; ++j)
+                            sum[i][j] += load_agent(
+                                part, q * SPLIT_BYTES + ((i * T::NB + j) * T::NT + t) * 16);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < T::MB; ++i)
+#pragma unroll
+                for (int j = 0; j < T::NB; ++j) acc[i][j] = sum[i][j];
         }
 
         // ---- epilogue: lane holds rows 4g..4g+3 of column l16 of each block.

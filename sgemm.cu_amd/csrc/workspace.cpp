@@ -1,0 +1,94 @@
+// workspace.cpp -- split-K workspace per stream and the split count rule.
+//
+// Split-K partials and the per-tile arrival counters live in device memory
+// owned by the library: one workspace per stream (launches on one stream are
+// ordered, so they can share it; launches on different streams get their own).
+// The counters are zeroed once when allocated and every split-K launch leaves
+// them zero again, so no memset is needed per launch (and none lands in a
+// captured graph).  A workspace that is outgrown is retired, not freed: a
+// captured graph may still hold its address, so it stays valid until the
+// process ends.  Growth is geometric, so the retired total stays below the
+// live workspace.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace sgemm_amd {
+
+namespace {
+
+struct Workspace {
+    void* partials = nullptr;
+    size_t partial_bytes = 0;
+    int* counters = nullptr;
+    int counter_count = 0;
+};
+
+std::mutex g_mu;
+std::map<hipStream_t, Workspace> g_ws;
+std::vector<void*> g_retired;  // outgrown buffers, kept for captured graphs
+
+int cu_count() {
+    static int cus = [] {
+        int dev = 0, n = 256;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+            return n;
+        (void)hipGetLastError();
+        return 256;
+    }();
+    return cus;
+}
+
+}  // namespace
+
+int splitk_count(long long tiles, int k, int bk) {
+    const long long target = 2LL * cu_count();
+    if (tiles >= target) return 1;
+    long long s = target / std::max(tiles, 1LL);
+    const long long max_by_k = std::max(1, k / (4 * bk));
+    s = std::min({s, max_by_k, 16LL});
+    return (int)std::max(1LL, s);
+}
+
+hipError_t splitk_workspace(hipStream_t stream, size_t partial_bytes, int tiles,
+                            float** partials, int** counters) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    Workspace& w = g_ws[stream];
+    if (w.partial_bytes < partial_bytes) {
+        const size_t bytes = std::max(partial_bytes + partial_bytes / 4, 2 * w.partial_bytes);
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return hipErrorOutOfMemory;
+        }
+        if (w.partials) g_retired.push_back(w.partials);
+        w.partials = p;
+        w.partial_bytes = bytes;
+    }
+    if (w.counter_count < tiles) {
+        const int n = std::max({tiles, 4096, 2 * w.counter_count});
+        int* c = nullptr;
+        if (hipMalloc(&c, (size_t)n * sizeof(int)) != hipSuccess) {
+            (void)hipGetLastError();
+            return hipErrorOutOfMemory;
+        }
+        if (hipMemset(c, 0, (size_t)n * sizeof(int)) != hipSuccess) {
+            (void)hipFree(c);
+            return hipErrorUnknown;
+        }
+        if (w.counters) g_retired.push_back(w.counters);
+        w.counters = c;
+        w.counter_count = n;
+    }
+    *partials = static_cast<float*>(w.partials);
+    *counters = w.counters;
+    return hipSuccess;
+}
+
+}  // namespace sgemm_amd

@@ -23,7 +23,7 @@ from gpu_util import Case, assert_ok, check, compact, download, log, run_form, u
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gemm_cases.json")
-N_F32_CONFIGS = 6
+N_F32_CONFIGS = 9
 
 
 @pytest.fixture(scope="module")
@@ -251,3 +251,49 @@ def test_sweep_shapes_sampled_fp64_and_full_vs_naive(m, n, k):
     diff = (dC - dC2).abs()
     bound = 2 * orc.fp32_tol_mult(k) * 1.001 * scale
     assert bool(torch.all(diff <= bound)), float((diff / bound).max())
+
+
+# ------------------------------------------------------------------- split-K
+SPLIT_CONFIGS = (6, 7, 8)
+
+
+@pytest.mark.parametrize("config", SPLIT_CONFIGS)
+def test_splitk_deterministic_and_in_tolerance(config):
+    """Split-K sums its partials in split order, so repeated launches give
+    identical bits; results meet the FP32 tolerance against FP64.  A second
+    problem of the same shape in between reuses the same partial slots, so a
+    reduction that read stale partials would fail its tolerance check."""
+    for (m, n, k) in [(1024, 1024, 1024), (300, 200, 5000), (129, 257, 777), (64, 64, 8192)]:
+        c = Case(m, n, k, seed=m + k, alpha=1.25, beta=0.5)
+        first = run_form(c, sa.FORM_F32, config)
+        assert_ok(check(c, first, orc.fp32_tol_mult(k)), f"split-K {(m, n, k)} config {config}")
+        other = Case(m, n, k, seed=m + k + 1000, alpha=1.0, beta=0.0)
+        out = run_form(other, sa.FORM_F32, config)
+        assert_ok(check(other, out, orc.fp32_tol_mult(k)), f"split-K 2nd {(m, n, k)} config {config}")
+        again = run_form(c, sa.FORM_F32, config)
+        assert np.array_equal(first, again), (m, n, k)
+
+
+def test_splitk_graph_replays_equal_eager():
+    """The per-tile counters reset themselves, so a captured split-K launch
+    replays correctly many times (no memset in the graph)."""
+    import torch
+    m, n, k = 512, 384, 4096
+    c = Case(m, n, k, seed=5, alpha=1.0, beta=0.0)
+    dA, dB, dC = c.device()
+    sa.run_config(0, 6, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m)  # allocates the workspace
+    torch.cuda.synchronize()
+    eager = download(dC).copy()
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        sa.run_config(0, 6, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, s)  # this stream's workspace
+        s.synchronize()
+        with torch.cuda.graph(g, stream=s):
+            sa.run_config(0, 6, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, torch.cuda.current_stream())
+    for rep in range(4):
+        dC.fill_(123.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(download(dC), eager), rep
