@@ -71,24 +71,60 @@ void ensure_env_table(Table& t) {
     if (p && *p) load_locked(t, p);
 }
 
-int heuristic(int form, int grouped, int m, int n) {
+int heuristic(int form, int grouped, int m, int n, int k) {
     if (form == kFormF32) {
         if (grouped) {
             if (m >= 96 && n >= 96) return 0;
             return m >= 96 ? 2 : 1;
         }
+        // Measured on MI355X (square sweep 1024..16384 and the odd shapes):
+        // 128x128x16 with three workgroups per CU (config 5) wins once there
+        // are four or more waves of 128x128 tiles, unless a short last wave
+        // of a few-wave grid idles most CUs; 128x128 with k-major A and two
+        // per CU (config 4) wins from two waves when the last wave is at
+        // least half full; everything smaller goes to 64x64 tiles with
+        // split-K (config 7), which splits k only when the grid has fewer
+        // than two workgroups per CU.  Short k (<= 256) is epilogue-bound:
+        // 128x64 (config 2).
         const long long t128 = (long long)div_up(m, 128) * div_up(n, 128);
-        if (t128 >= 512) return 0;
-        const long long t64 = (long long)div_up(m, 128) * div_up(n, 64);
-        return t64 >= 512 ? 2 : 1;
+        if (k <= 256 && t128 >= 512) return 2;
+        if (t128 >= 1024) {
+            const long long rem = t128 % 768, waves = div_up(t128, 768LL);
+            return (waves <= 3 && rem > 0 && rem < 128) ? 7 : 5;
+        }
+        if (t128 >= 512) {
+            const long long rem = t128 % 512;
+            return (rem == 0 || rem >= 256) ? 4 : 7;
+        }
+        return 7;
     }
     if (grouped) return (m >= 96 && n >= 96) ? 0 : 1;
-    // 256x256 tiles with 8 waves (config 2) run fastest from ~3k^2 up; below
-    // that they leave CUs idle.
-    const long long t256 = (long long)div_up(m, 256) * div_up(n, 256);
-    if (t256 >= 128) return 2;
-    const long long t128 = (long long)div_up(m, 128) * div_up(n, 128);
-    return t128 >= 256 ? 0 : 1;
+    // BF16x3: all configs but 5 (256x128, two per CU) and 7 (64x64 with
+    // split-K, two per CU) hold one workgroup per CU, and every tile of a
+    // config takes the same time, so the launch takes ceil(workgroups /
+    // slots) rounds.  Cost per config = rounds * (tile area per CU) / splits /
+    // (relative rate measured at 8192^3); the cheapest wins.  Config 7 splits
+    // k when its grid is short of two workgroups per CU (splitk_count).
+    struct Cand { int id, bm, bn, bk, per_cu; double rate; bool split; };
+    static const Cand cands[] = {
+        {2, 256, 256, 32, 1, 1.00, false}, {5, 256, 128, 16, 2, 0.92, false},
+        {3, 256, 128, 32, 1, 0.69, false}, {0, 128, 128, 32, 1, 0.50, false},
+        {7, 64, 64, 64, 2, 0.47, true},
+    };
+    int best = 7;
+    double best_cost = 0;
+    for (const Cand& c : cands) {
+        const long long tiles = (long long)div_up(m, c.bm) * div_up(n, c.bn);
+        const long long splits = c.split ? splitk_count(tiles, k, c.bk) : 1;
+        const long long slots = (long long)device_cu_count() * c.per_cu;
+        const double cost =
+            (double)div_up(tiles * splits, slots) * c.per_cu * c.bm * c.bn / c.rate / splits;
+        if (best_cost == 0 || cost < best_cost) {
+            best = c.id;
+            best_cost = cost;
+        }
+    }
+    return best;
 }
 
 int lookup(int form, int grouped, int m, int n, int k) {
@@ -100,7 +136,7 @@ int lookup(int form, int grouped, int m, int n, int k) {
         const int c = it->second.config;
         if (c >= 0 && c < config_count(form) && (!grouped || grouped_capable(form, c))) return c;
     }
-    return heuristic(form, grouped, m, n);
+    return heuristic(form, grouped, m, n, k);
 }
 
 int load_locked(Table& t, const char* path) {

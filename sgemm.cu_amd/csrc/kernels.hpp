@@ -15,9 +15,18 @@ struct ConfigInfo {
     int splitk;      // 1: the k range is split across workgroups (splitk_count)
 };
 
+// Compute units of the current device (queried once per process).
+int device_cu_count();
+
 // Number of k-splits for a tiles-sized grid: enough workgroups for two per
 // CU, each split keeping at least four k-tiles, at most 16 splits.
 int splitk_count(long long tiles, int k, int bk);
+
+// Split plan for a tiles-sized grid of BK-deep tiles whose split partials
+// take split_bytes per tile: sets sk->splits (1 = no split) and kchunk, and
+// for a split grid the workspace pointers.
+hipError_t splitk_plan(hipStream_t stream, long long tiles, int k, int bk, size_t split_bytes,
+                       SplitK* sk);
 
 // Per-stream split-K workspace (partials + zeroed per-tile counters), grown on
 // demand; the first call on a stream, or a larger problem, allocates.

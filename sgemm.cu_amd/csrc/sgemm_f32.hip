@@ -47,16 +47,10 @@ static hipError_t launch_tile(const Problem& p, hipStream_t s) {
     if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
     SplitK sk{nullptr, nullptr, 1, p.k};
     if (ALLOW_SPLIT) {
-        sk.splits = splitk_count(tiles, p.k, T::BK);
-        sk.kchunk = div_up(div_up(p.k, sk.splits), T::BK) * T::BK;
-        sk.splits = div_up(p.k, sk.kchunk);  // every split gets k
-    }
-    const bool split = ALLOW_SPLIT && sk.splits > 1;
-    if (split) {
-        const size_t bytes = (size_t)tiles * sk.splits * T::MB * T::NB * T::NT * 16;
-        hipError_t e = splitk_workspace(s, bytes, (int)tiles, &sk.partials, &sk.counters);
+        hipError_t e = splitk_plan(s, tiles, p.k, T::BK, (size_t)T::MB * T::NB * T::NT * 16, &sk);
         if (e != hipSuccess) return e;
     }
+    const bool split = ALLOW_SPLIT && sk.splits > 1;
     const dim3 grid((unsigned)(tiles * sk.splits));
     (void)hipGetLastError();  // the status read below must belong to this launch
     constexpr bool S = ALLOW_SPLIT;

@@ -26,6 +26,8 @@
 //    data-dependent branch, unlike 128x128x8.cuh:344-358).
 #pragma once
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace sgemm_amd {
@@ -140,7 +142,9 @@ struct F32TileRunner {
 #pragma unroll
             for (int r = 0; r < T::B_VEC; ++r) rb[r] = bload4<VEC>(br, b_off[r]);
         };
-        auto sstore = [&](float* stage, int k0) {
+        // MASK: zero k >= K (only the last k-tile can reach past K).
+        auto sstore_impl = [&](float* stage, int k0, auto mask) {
+            constexpr bool MASK = decltype(mask)::value;
             float* As = stage;
             float* Bs = stage + T::A_FLOATS;
             if constexpr (T::AT) {
@@ -151,7 +155,7 @@ struct F32TileRunner {
                     for (int i = 0; i < 4; ++i) {
                         f32x4 v;  // row 4*m4 + i, k = 4*at_k4 .. +3
 #pragma unroll
-                        for (int kk = 0; kk < 4; ++kk) v[kk] = kk < ka ? ra[4 * sb + kk][i] : 0.f;
+                        for (int kk = 0; kk < 4; ++kk) v[kk] = !MASK || kk < ka ? ra[4 * sb + kk][i] : 0.f;
                         const int row = 4 * (at_m4 + sb * (T::NT / AS)) + i;
                         *reinterpret_cast<f32x4*>(&As[row * T::SA + 4 * at_k4]) = v;
                     }
@@ -159,7 +163,7 @@ struct F32TileRunner {
 #pragma unroll
                 for (int r = 0; r < T::A_VEC; ++r) {
                     const f32x4 v =
-                        (k0 + a_k + r * T::A_KSTEP < K) ? ra[r] : f32x4{0.f, 0.f, 0.f, 0.f};
+                        !MASK || (k0 + a_k + r * T::A_KSTEP < K) ? ra[r] : f32x4{0.f, 0.f, 0.f, 0.f};
                     *reinterpret_cast<f32x4*>(&As[(a_k + r * T::A_KSTEP) * T::SA + a_m]) = v;
                 }
             }
@@ -168,9 +172,13 @@ struct F32TileRunner {
             for (int r = 0; r < T::B_VEC; ++r) {
                 f32x4 v;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) v[e] = e < kv ? rb[r][e] : 0.f;
+                for (int e = 0; e < 4; ++e) v[e] = !MASK || e < kv ? rb[r][e] : 0.f;
                 *reinterpret_cast<f32x4*>(&Bs[(b_n + r * T::B_NSTEP) * T::SB + b_k]) = v;
             }
+        };
+        auto sstore = [&](float* stage, int k0) {
+            if (k0 + BK <= K) sstore_impl(stage, k0, std::false_type{});
+            else sstore_impl(stage, k0, std::true_type{});
         };
 
         f32x4 acc[T::MB][T::NB];

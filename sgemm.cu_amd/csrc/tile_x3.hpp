@@ -83,13 +83,22 @@ __device__ inline f32x4 x3_load4(const float* p, int valid) {
     return v;
 }
 
-// hi/lo split of four values (round to nearest even both times).
+// hi/lo split of four values (round to nearest even both times), two at a
+// time so each rounding is one v_cvt_pk_bf16_f32 for a pair.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
 __device__ inline void split4(f32x4 x, bf16x4_t& hi, bf16x4_t& lo) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const __bf16 h = static_cast<__bf16>(x[i]);
-        hi[i] = h;
-        lo[i] = static_cast<__bf16>(x[i] - static_cast<float>(h));
+    for (int p = 0; p < 2; ++p) {
+        const f32x2 v = {x[2 * p], x[2 * p + 1]};
+        const bf16x2_t h = __builtin_convertvector(v, bf16x2_t);
+        const f32x2 r = v - __builtin_convertvector(h, f32x2);  // exact in FP32
+        const bf16x2_t l = __builtin_convertvector(r, bf16x2_t);
+        hi[2 * p] = h[0];
+        hi[2 * p + 1] = h[1];
+        lo[2 * p] = l[0];
+        lo[2 * p + 1] = l[1];
     }
 }
 
@@ -97,7 +106,11 @@ template <class T, bool VEC>
 struct X3TileRunner {
     static constexpr int BM = T::BM, BN = T::BN, BK = T::BK;
 
-    __device__ static void run(const Problem& P, int tm, int tn, char* smem) {
+    // All threads of the workgroup must call this together.  With `sk`, only
+    // k in [split * kchunk, (split + 1) * kchunk) is accumulated and the
+    // partials are reduced across splits (common.hpp).
+    __device__ static void run(const Problem& P, int tm, int tn, char* smem,
+                               const SplitK* sk = nullptr, int split = 0) {
         const int t = threadIdx.x;
         const int lane = t & 63, wave = t >> 6;
         const int wm = wave % T::WM, wn = wave / T::WM;
@@ -141,7 +154,9 @@ struct X3TileRunner {
 #pragma unroll
             for (int r = 0; r < T::B_VEC; ++r) rb[r] = bload4<VEC>(br, b_off[r]);
         };
-        auto sstore = [&](char* stage, int k0) {
+        // MASK: zero k >= K (only the last k-tile can reach past K).
+        auto sstore_impl = [&](char* stage, int k0, auto mask) {
+            constexpr bool MASK = decltype(mask)::value;
             char* Ahi = stage;
             char* Alo = stage + T::A_BYTES;
             char* Bhi = stage + 2 * T::A_BYTES;
@@ -154,7 +169,7 @@ struct X3TileRunner {
                     // row 4*m4 + i gets k = 4*a_k4 .. +3
                     f32x4 col;
 #pragma unroll
-                    for (int kk = 0; kk < 4; ++kk) col[kk] = kk < ka ? ra[r][kk][i] : 0.f;
+                    for (int kk = 0; kk < 4; ++kk) col[kk] = !MASK || kk < ka ? ra[r][kk][i] : 0.f;
                     bf16x4_t hi, lo;
                     split4(col, hi, lo);
                     const int off = (4 * (a_m4 + r * T::A_MSTEP) + i) * T::RS + 8 * a_k4;
@@ -166,13 +181,17 @@ struct X3TileRunner {
             for (int r = 0; r < T::B_VEC; ++r) {
                 f32x4 v;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) v[e] = e < kb ? rb[r][e] : 0.f;
+                for (int e = 0; e < 4; ++e) v[e] = !MASK || e < kb ? rb[r][e] : 0.f;
                 bf16x4_t hi, lo;
                 split4(v, hi, lo);
                 const int off = (b_n + r * T::B_NSTEP) * T::RS + 2 * b_k;
                 *reinterpret_cast<bf16x4_t*>(Bhi + off) = hi;
                 *reinterpret_cast<bf16x4_t*>(Blo + off) = lo;
             }
+        };
+        auto sstore = [&](char* stage, int k0) {
+            if (k0 + BK <= K) sstore_impl(stage, k0, std::false_type{});
+            else sstore_impl(stage, k0, std::true_type{});
         };
 
         using Acc = typename std::conditional<T::MF == 16, f32x4, f32x16>::type;
@@ -190,10 +209,12 @@ struct X3TileRunner {
         // consecutive k at kk + 8 * (lane / MF).
         const int lrow = lane % T::MF, lk = 8 * (lane / T::MF);
 
-        const int ktiles = div_up(K, BK);
+        const int kbeg = sk ? split * sk->kchunk : 0;
+        const int kend = sk ? min(K, kbeg + sk->kchunk) : K;
+        const int ktiles = kend > kbeg ? div_up(kend - kbeg, BK) : 0;
         if (ktiles > 0) {
-            gload(0);
-            sstore(smem, 0);
+            gload(kbeg);
+            sstore(smem, kbeg);
             __syncthreads();
         }
         for (int kt = 0; kt < ktiles; ++kt) {
@@ -203,7 +224,8 @@ struct X3TileRunner {
             const char* Bhi = stage + 2 * T::A_BYTES;
             const char* Blo = Bhi + T::B_BYTES;
             const bool more = kt + 1 < ktiles;
-            if (more) gload((kt + 1) * BK);
+            const int knext = kbeg + (kt + 1) * BK;
+            if (more) gload(knext);
 #pragma unroll
             for (int kk = 0; kk < BK; kk += T::KSTEP) {
                 bf16x8_t ah[T::MB], al[T::MB], bh[T::NB], bl[T::NB];
@@ -246,9 +268,78 @@ struct X3TileRunner {
                 // Stage the next k-tile (wait on its loads, split, LDS stores)
                 // behind the first k-step's MFMAs, or after the last one.
                 constexpr int kStoreAt = T::MID ? 0 : BK - T::KSTEP;
-                if (kk == kStoreAt && more) sstore(smem + ((kt + 1) & 1) * T::STAGE, (kt + 1) * BK);
+                if (kk == kStoreAt && more) sstore(smem + ((kt + 1) & 1) * T::STAGE, knext);
             }
             __syncthreads();
+        }
+
+        if (sk) {
+            // Fold the cross-term accumulators in, dump this split's sums in
+            // lane order, and let the last workgroup of the tile add all
+            // splits in split order (its own from registers).
+            if constexpr (T::DUAL) {
+#pragma unroll
+                for (int i = 0; i < T::MB; ++i)
+#pragma unroll
+                    for (int j = 0; j < T::NB; ++j) {
+                        acc[i][j] += crs[i][j];
+                        crs[i][j] = Acc{};
+                    }
+            }
+            constexpr int NV = sizeof(Acc) / 16;  // float4 pieces per accumulator
+            constexpr int NACC = T::MB * T::NB * NV;
+            constexpr int SPLIT_BYTES = NACC * T::NT * 16;
+            const int tile = tm * P.tiles_n + tn;
+            const __amdgpu_buffer_rsrc_t part =
+                make_rsrc(sk->partials + (size_t)tile * sk->splits * NACC * T::NT * 4,
+                          (long long)sk->splits * SPLIT_BYTES);
+            auto piece = [&](int i, int j, int v) -> f32x4 {
+                return f32x4{acc[i][j][4 * v], acc[i][j][4 * v + 1], acc[i][j][4 * v + 2],
+                             acc[i][j][4 * v + 3]};
+            };
+            auto off = [&](int i, int j, int v) { return (((i * T::NB + j) * NV + v) * T::NT + t) * 16; };
+#pragma unroll
+            for (int i = 0; i < T::MB; ++i)
+#pragma unroll
+                for (int j = 0; j < T::NB; ++j)
+#pragma unroll
+                    for (int v = 0; v < NV; ++v)
+                        store_agent(part, split * SPLIT_BYTES + off(i, j, v), piece(i, j, v));
+            if (!splitk_arrive(*sk, tile, reinterpret_cast<int*>(smem))) return;
+            Acc sum[T::MB][T::NB];
+#pragma unroll
+            for (int i = 0; i < T::MB; ++i)
+#pragma unroll
+                for (int j = 0; j < T::NB; ++j)
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        const f32x4 x = split == 0 ? piece(i, j, v) : load_agent(part, off(i, j, v));
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) sum[i][j][4 * v + r] = x[r];
+                    }
+            for (int q = 1; q < sk->splits; ++q) {
+                if (q == split) {
+#pragma unroll
+                    for (int i = 0; i < T::MB; ++i)
+#pragma unroll
+                        for (int j = 0; j < T::NB; ++j) sum[i][j] += acc[i][j];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < T::MB; ++i)
+#pragma unroll
+                        for (int j = 0; j < T::NB; ++j)
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) {
+                                const f32x4 x = load_agent(part, q * SPLIT_BYTES + off(i, j, v));
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) sum[i][j][4 * v + r] += x[r];
+                            }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < T::MB; ++i)
+#pragma unroll
+                for (int j = 0; j < T::NB; ++j) acc[i][j] = sum[i][j];
         }
 
         // Epilogue.  16x16 block: lane holds rows 4*(lane/16) + 0..3 of column

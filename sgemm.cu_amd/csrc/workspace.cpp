@@ -5,7 +5,10 @@
 // ordered, so they can share it; launches on different streams get their own).
 // The counters are zeroed once when allocated and every split-K launch leaves
 // them zero again, so no memset is needed per launch (and none lands in a
-// captured graph).  A workspace that is outgrown is retired, not freed: a
+// captured graph).  A stream that is being captured never allocates: a
+// split-K launch whose workspace would have to grow runs unsplit instead, so
+// warm a stream up before capturing on it to keep the split.  A workspace
+// that is outgrown is retired, not freed: a
 // captured graph may still hold its address, so it stays valid until the
 // process ends.  Growth is geometric, so the retired total stays below the
 // live workspace.
@@ -33,7 +36,9 @@ std::mutex g_mu;
 std::map<hipStream_t, Workspace> g_ws;
 std::vector<void*> g_retired;  // outgrown buffers, kept for captured graphs
 
-int cu_count() {
+}  // namespace
+
+int device_cu_count() {
     static int cus = [] {
         int dev = 0, n = 256;
         if (hipGetDevice(&dev) == hipSuccess &&
@@ -45,10 +50,8 @@ int cu_count() {
     return cus;
 }
 
-}  // namespace
-
 int splitk_count(long long tiles, int k, int bk) {
-    const long long target = 2LL * cu_count();
+    const long long target = 2LL * device_cu_count();
     if (tiles >= target) return 1;
     long long s = target / std::max(tiles, 1LL);
     const long long max_by_k = std::max(1, k / (4 * bk));
@@ -56,10 +59,37 @@ int splitk_count(long long tiles, int k, int bk) {
     return (int)std::max(1LL, s);
 }
 
+hipError_t splitk_plan(hipStream_t stream, long long tiles, int k, int bk, size_t split_bytes,
+                       SplitK* sk) {
+    sk->partials = nullptr;
+    sk->counters = nullptr;
+    sk->splits = splitk_count(tiles, k, bk);
+    sk->kchunk = div_up(div_up(k, sk->splits), bk) * bk;
+    sk->splits = div_up(k, sk->kchunk);  // every split gets some k
+    if (sk->splits <= 1) return hipSuccess;
+    const hipError_t e = splitk_workspace(stream, (size_t)tiles * sk->splits * split_bytes,
+                                          (int)tiles, &sk->partials, &sk->counters);
+    if (e == hipErrorStreamCaptureUnsupported) {
+        // Capturing, and this stream's workspace is too small: run unsplit
+        // rather than allocate inside the capture.
+        sk->splits = 1;
+        sk->kchunk = k;
+        sk->partials = nullptr;
+        sk->counters = nullptr;
+        return hipSuccess;
+    }
+    return e;
+}
+
 hipError_t splitk_workspace(hipStream_t stream, size_t partial_bytes, int tiles,
                             float** partials, int** counters) {
     std::lock_guard<std::mutex> lock(g_mu);
     Workspace& w = g_ws[stream];
+    if (w.partial_bytes < partial_bytes || w.counter_count < tiles) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(stream, &cs) != hipSuccess) (void)hipGetLastError();
+        if (cs != hipStreamCaptureStatusNone) return hipErrorStreamCaptureUnsupported;
+    }
     if (w.partial_bytes < partial_bytes) {
         const size_t bytes = std::max(partial_bytes + partial_bytes / 4, 2 * w.partial_bytes);
         void* p = nullptr;

@@ -21,7 +21,7 @@ from gpu_util import Case, assert_ok, check, compact, download, log, run_form, u
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gemm_cases.json")
-N_X3_CONFIGS = 6
+N_X3_CONFIGS = 8
 
 
 def test_config_table():
@@ -129,3 +129,33 @@ def test_error_within_fp32_bound_at_8192():
     log(f"bf16x3 8192^3 wide sampled max comp err {e.max():.3e} (fp32 bound "
         f"{orc.fp32_tol_mult(n):.3e})")
     assert np.all(e <= orc.fp32_tol_mult(n))
+
+
+# ------------------------------------------------------------------- split-K
+@pytest.mark.parametrize("config", (6, 7))
+def test_splitk_deterministic_and_in_tolerance(config):
+    """Split-K configs (tiles of configs 0 and 1): partials summed in split
+    order, so repeated launches are bit-identical; a different problem of the
+    same shape in between reuses the partial slots (a stale read would fail
+    its tolerance check)."""
+    for (m, n, k) in [(256, 256, 8192), (300, 200, 5000), (129, 257, 777), (1024, 64, 4096)]:
+        c = Case(m, n, k, seed=m + k, alpha=1.25, beta=0.5)
+        first = run_form(c, sa.FORM_BF16X3, config)
+        assert_ok(check(c, first, orc.bf16x3_tol_mult(k)), f"split-K {(m, n, k)} config {config}")
+        other = Case(m, n, k, kind="wide", seed=m + k + 1000, alpha=1.0, beta=0.0)
+        out = run_form(other, sa.FORM_BF16X3, config)
+        assert_ok(check(other, out, orc.bf16x3_tol_mult(k)),
+                  f"split-K 2nd {(m, n, k)} config {config}")
+        again = run_form(c, sa.FORM_BF16X3, config)
+        assert np.array_equal(first, again), (m, n, k)
+
+
+def test_splitk_integer_kat_bit_exact():
+    """Integer inputs are BF16-exact and every partial sum is an exact
+    integer, so the split result equals the unsplit one bit for bit."""
+    m, n, k = 96, 80, 4096
+    c = Case(m, n, k, kind="int", seed=9, alpha=1.5, beta=0.5)
+    a = run_form(c, sa.FORM_BF16X3, 0)
+    b = run_form(c, sa.FORM_BF16X3, 6)
+    assert np.array_equal(a, b)
+    assert_ok(check(c, b, orc.bf16x3_tol_mult(k)), "split-K integer KAT")

@@ -297,3 +297,26 @@ def test_splitk_graph_replays_equal_eager():
         g.replay()
         torch.cuda.synchronize()
         assert np.array_equal(download(dC), eager), rep
+
+
+def test_splitk_capture_without_warmup_runs_unsplit():
+    """A split-K launch captured on a stream that has no workspace yet does
+    not allocate inside the capture: it runs the unsplit tile (config 0's
+    kernel), so each replay equals config 0's eager result bit for bit."""
+    import torch
+    m, n, k = 512, 384, 4096
+    c = Case(m, n, k, seed=11, alpha=1.0, beta=0.0)
+    dA, dB, dC = c.device()
+    sa.run_config(0, 0, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m)
+    torch.cuda.synchronize()
+    unsplit = download(dC).copy()
+    fresh = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=fresh):
+        sa.run_config(0, 6, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, torch.cuda.current_stream())
+    for rep in range(3):
+        dC.fill_(-1.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(download(dC), unsplit), rep
