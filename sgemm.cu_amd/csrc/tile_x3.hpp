@@ -39,14 +39,16 @@ typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 // registers and so allows larger wave tiles.  MID stages the next k-tile
 // right after the first k-step's MFMAs instead of after the last one; it pays
 // when two independent workgroups share a CU and hurts a lone wave per SIMD,
-// whose loads then have less time to land.
+// whose loads then have less time to land.  SCHED applies x3_schedule's
+// interleaving hints to the full k-tiles.
 template <int BM_, int BN_, int BK_, int WM_, int WN_, int MF_ = 16, bool DUAL_ = true,
-          bool MID_ = false>
+          bool MID_ = false, bool SCHED_ = false>
 struct X3Tile {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
     static constexpr int MF = MF_;
     static constexpr bool DUAL = DUAL_;
     static constexpr bool MID = MID_;
+    static constexpr bool SCHED = SCHED_;
     // Waves per SIMD the kernel's registers must allow: two independent
     // workgroups per CU for the MID configs, else whatever the tile needs.
     static constexpr int WPE = MID_ ? 2 : 1;
@@ -99,6 +101,39 @@ __device__ inline void split4(f32x4 x, bf16x4_t& hi, bf16x4_t& lo) {
         hi[2 * p + 1] = h[1];
         lo[2 * p] = l[0];
         lo[2 * p + 1] = l[1];
+    }
+}
+
+// Instruction-group order for the compiler's scheduler in a full k-tile
+// (__builtin_amdgcn_sched_group_barrier: mask, count, sync id).  Masks: 0x008
+// MFMA, 0x002 VALU, 0x020 VMEM read, 0x100 DS read, 0x200 DS write.  The next
+// k-tile's global loads go first; each k-step's operand reads precede its
+// MFMAs; during the last k-step every MFMA is followed by a share of the
+// next k-tile's split (VALU) and one of its LDS stores, so the matrix core
+// keeps receiving work while the VALU and LDS ports are busy.  (Measured on
+// the 256x256 tile: a few percent over the unhinted schedule; on the
+// one-wave-per-SIMD tiles the hints lose, so only config 2 uses them.)
+template <class T>
+__device__ inline void x3_schedule() {
+    constexpr int STEPS = T::BK / T::KSTEP;
+    constexpr int MF_STEP = T::MB * T::NB * 3;                // MFMAs per k-step
+    constexpr int RD_STEP = (T::MB + T::NB) * 2;              // b128 operand reads per k-step
+    constexpr int GLOADS = T::A_SUB * 4 + T::B_VEC;           // 16-byte global loads
+    constexpr int WRITES = T::A_SUB * 4 * 2 + T::B_VEC * 2;   // LDS stores of hi and lo
+    constexpr int VALU = (T::A_SUB * 16 + T::B_VEC * 4) * 3;  // split: ~3 per element
+    constexpr int VPER = (VALU + MF_STEP - 1) / MF_STEP;
+    __builtin_amdgcn_sched_group_barrier(0x020, GLOADS, 0);
+#pragma unroll
+    for (int s = 0; s < STEPS; ++s) {
+        __builtin_amdgcn_sched_group_barrier(0x100, RD_STEP, 0);
+#pragma unroll
+        for (int i = 0; i < MF_STEP; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            if (s + 1 == STEPS) {
+                __builtin_amdgcn_sched_group_barrier(0x002, VPER, 0);
+                if (i < WRITES) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+            }
+        }
     }
 }
 
@@ -217,15 +252,23 @@ struct X3TileRunner {
             sstore(smem, kbeg);
             __syncthreads();
         }
-        for (int kt = 0; kt < ktiles; ++kt) {
+        // One k-tile: MFMAs on the current stage while the next k-tile is
+        // loaded, split and stored to the other stage.  NEXT_FULL: a next
+        // k-tile exists and lies entirely below K, so the body needs no
+        // masking and no branch -- one basic block, which the scheduling
+        // hints below can interleave.
+        auto iteration = [&](int kt, auto next_full_tag) {
+            constexpr bool NEXT_FULL = decltype(next_full_tag)::value;
             const char* stage = smem + (kt & 1) * T::STAGE;
+            char* next_stage = smem + ((kt + 1) & 1) * T::STAGE;
             const char* Ahi = stage;
             const char* Alo = stage + T::A_BYTES;
             const char* Bhi = stage + 2 * T::A_BYTES;
             const char* Blo = Bhi + T::B_BYTES;
-            const bool more = kt + 1 < ktiles;
+            const bool more = NEXT_FULL || kt + 1 < ktiles;
             const int knext = kbeg + (kt + 1) * BK;
             if (more) gload(knext);
+            if constexpr (NEXT_FULL && !T::SCHED) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int kk = 0; kk < BK; kk += T::KSTEP) {
                 bf16x8_t ah[T::MB], al[T::MB], bh[T::NB], bl[T::NB];
@@ -268,8 +311,35 @@ struct X3TileRunner {
                 // Stage the next k-tile (wait on its loads, split, LDS stores)
                 // behind the first k-step's MFMAs, or after the last one.
                 constexpr int kStoreAt = T::MID ? 0 : BK - T::KSTEP;
-                if (kk == kStoreAt && more) sstore(smem + ((kt + 1) & 1) * T::STAGE, knext);
+                if (kk == kStoreAt) {
+                    if constexpr (NEXT_FULL) {
+                        // Without scheduling hints, keep the staging code a
+                        // region of its own (as a branch would): letting the
+                        // scheduler mix it into the MFMA section raises
+                        // register pressure past the budget.
+                        if constexpr (!T::SCHED) __builtin_amdgcn_sched_barrier(0);
+                        sstore_impl(next_stage, knext, std::false_type{});
+                        if constexpr (!T::SCHED) __builtin_amdgcn_sched_barrier(0);
+                    } else if (more) {
+                        sstore(next_stage, knext);
+                    }
+                }
             }
+            if constexpr (NEXT_FULL && T::SCHED) x3_schedule<T>();
+        };
+        // k-tiles whose successor is full run the branch-free body; the
+        // last one or two run the general body.  (Two loops in sequence, not
+        // one loop with two bodies, so the accumulators never merge from two
+        // paths -- that doubles their registers.)
+        const int kfull = K - kbeg >= BK ? (K - kbeg) / BK : 0;  // full k-tiles from kbeg
+        const int nfast = min(ktiles, kfull) - 1;              // iterations with a full successor
+        int kt = 0;
+        for (; kt < nfast; ++kt) {
+            iteration(kt, std::true_type{});
+            __syncthreads();
+        }
+        for (; kt < ktiles; ++kt) {
+            iteration(kt, std::false_type{});
             __syncthreads();
         }
 
@@ -389,7 +459,7 @@ struct X3TileRunner {
 // Config ids of the BF16x3 form (the order is the ABI's config id).
 using X3_128x128 = X3Tile<128, 128, 32, 2, 2>;
 using X3_64x64 = X3Tile<64, 64, 64, 2, 2>;
-using X3_256x256_w8 = X3Tile<256, 256, 32, 2, 4, 32, false>;
+using X3_256x256_w8 = X3Tile<256, 256, 32, 2, 4, 32, false, false, true>;
 using X3_256x128_w4 = X3Tile<256, 128, 32, 2, 2, 32, false>;
 using X3_128x128_m32 = X3Tile<128, 128, 32, 2, 2, 32, true>;
 using X3_256x128_k16 = X3Tile<256, 128, 16, 2, 2, 32, false, true>;
