@@ -37,11 +37,14 @@ namespace sgemm_amd {
 // and transposes it in registers; in exchange every A fragment is one
 // ds_read_b128 (four MFMA steps) instead of four 4-byte reads.
 // WPE_: waves per SIMD the register allocation must allow.
-template <int BM_, int BN_, int BK_, int WM_, int WN_, bool AT_ = false, int WPE_ = 1>
+// SCHED_: scheduling-hint pattern for full k-tiles (0: none; f32_schedule).
+template <int BM_, int BN_, int BK_, int WM_, int WN_, bool AT_ = false, int WPE_ = 1,
+          int SCHED_ = 0>
 struct F32Tile {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
     static constexpr bool AT = AT_;
     static constexpr int WPE = WPE_;
+    static constexpr int SCHED = SCHED_;
     static constexpr int NT = 64 * WM * WN;
     static constexpr int TM = BM / WM, TN = BN / WN;  // wave tile
     static constexpr int MB = TM / 16, NB = TN / 16;  // 16x16 MFMA blocks per wave
@@ -75,6 +78,37 @@ __device__ inline f32x4 load4(const float* p, int valid) {
     if (valid > 2) v[2] = p[2];
     if (valid > 3) v[3] = p[3];
     return v;
+}
+
+// Instruction-group order for the compiler's scheduler in a full k-tile
+// (__builtin_amdgcn_sched_group_barrier: mask, count, sync id; 0x008 MFMA,
+// 0x002 VALU, 0x020 VMEM read, 0x100 DS read, 0x200 DS write).  Pattern 1:
+// global loads first, all operand reads, then the MFMAs with one LDS store of
+// the next k-tile after each of the later ones.  Pattern 2: as 1, but the
+// operand reads are spread over the first MFMAs.
+template <class T>
+__device__ inline void f32_schedule() {
+    constexpr int NM = (T::BK / 16) * 4 * T::MB * T::NB;            // MFMAs
+    constexpr int GLOADS = T::A_VEC + T::B_VEC;                       // 16-byte loads
+    constexpr int READS = (T::BK / 16) * ((T::AT ? 1 : 2) * T::MB + T::NB);
+    constexpr int WRITES = T::A_VEC + T::B_VEC;                       // ds_write_b128
+    constexpr int WSTART = NM / 4;
+    __builtin_amdgcn_sched_group_barrier(0x020, GLOADS, 0);
+    if constexpr (T::SCHED == 1) __builtin_amdgcn_sched_group_barrier(0x100, READS, 0);
+    int w = 0, r = 0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        if (T::SCHED == 2 && r < READS && m % 2 == 0) {
+            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+            r += 2;
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (m >= WSTART && w < WRITES) {
+            __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+            ++w;
+        }
+    }
 }
 
 template <class T, bool VEC>
@@ -195,12 +229,19 @@ struct F32TileRunner {
             sstore(smem, kbeg);
             __syncthreads();
         }
-        for (int kt = 0; kt < ktiles; ++kt) {
+        // One k-tile: MFMAs out of the current stage while the next k-tile
+        // is loaded and stored to the other one.  NEXT_FULL: a next k-tile
+        // exists and lies entirely below K, so the body has no masking and
+        // no branch.
+        auto iteration = [&](int kt, auto next_full_tag) {
+            constexpr bool NEXT_FULL = decltype(next_full_tag)::value;
             const float* As = smem + (kt & 1) * T::STAGE;
             const float* Bs = As + T::A_FLOATS;
-            const bool more = kt + 1 < ktiles;
+            float* next_stage = smem + ((kt + 1) & 1) * T::STAGE;
+            const bool more = NEXT_FULL || kt + 1 < ktiles;
             const int knext = kbeg + (kt + 1) * BK;
             if (more) gload(knext);
+            if constexpr (NEXT_FULL && T::SCHED == 0) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int kk = 0; kk < BK; kk += 16) {
                 // Lane group g supplies k = kk + 4g + s at MFMA step s, for A
@@ -233,8 +274,30 @@ struct F32TileRunner {
                 // Stage the next k-tile behind the first k-step's MFMAs, so
                 // the wait on its loads, the masking and the LDS stores
                 // overlap matrix-core work instead of following all of it.
-                if (kk == 0 && more) sstore(smem + ((kt + 1) & 1) * T::STAGE, knext);
+                if (kk == 0) {
+                    if constexpr (NEXT_FULL) {
+                        if constexpr (T::SCHED == 0) __builtin_amdgcn_sched_barrier(0);
+                        sstore_impl(next_stage, knext, std::false_type{});
+                        if constexpr (T::SCHED == 0) __builtin_amdgcn_sched_barrier(0);
+                    } else if (more) {
+                        sstore(next_stage, knext);
+                    }
+                }
             }
+            if constexpr (NEXT_FULL && T::SCHED != 0) f32_schedule<T>();
+        };
+        // k-tiles whose successor is full run the branch-free body, the last
+        // one or two the general body (two loops in sequence, so the
+        // accumulators never merge from two paths).
+        const int kfull = K - kbeg >= BK ? (K - kbeg) / BK : 0;
+        const int nfast = min(ktiles, kfull) - 1;
+        int kt = 0;
+        for (; kt < nfast; ++kt) {
+            iteration(kt, std::true_type{});
+            __syncthreads();
+        }
+        for (; kt < ktiles; ++kt) {
+            iteration(kt, std::false_type{});
             __syncthreads();
         }
 
@@ -324,7 +387,7 @@ using F32_64x64 = F32Tile<64, 64, 32, 2, 2>;
 using F32_128x64 = F32Tile<128, 64, 32, 2, 2>;
 using F32_256x128 = F32Tile<256, 128, 16, 2, 2>;
 using F32_128x128_at = F32Tile<128, 128, 32, 2, 2, true, 2>;
-using F32_128x128_k16 = F32Tile<128, 128, 16, 2, 2, false, 3>;
+using F32_128x128_k16 = F32Tile<128, 128, 16, 2, 2, false, 3, 1>;
 
 template <class T>
 struct TypeTag {
