@@ -298,7 +298,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+              "(multi-GPU sharding is not implemented)", file=sys.stderr)
         sys.exit(2)
     import torch
     import sgemm_amd as sa

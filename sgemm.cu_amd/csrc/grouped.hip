@@ -60,7 +60,7 @@ static_assert(X3_64x64::LDS_BYTES <= kGroupedLdsX3, "X3 grouped LDS");
 
 template <class T, bool VEC>
 __device__ inline void run_tile_f32(const Problem& P, int tm, int tn, char* smem) {
-    F32TileRunner<T, VEC>::run(P, tm, tn, reinterpret_cast<float*>(smem));
+    F32TileRunner<T, VEC, false>::run(P, tm, tn, reinterpret_cast<float*>(smem));
 }
 
 template <int FORM>
@@ -97,10 +97,10 @@ __global__ __launch_bounds__(256) void k_grouped(const GroupHeader* __restrict__
             }
         } else {
             switch (P.cfg) {
-                case 0: vec ? X3TileRunner<X3_128x128, true>::run(P, tm, tn, smem)
-                            : X3TileRunner<X3_128x128, false>::run(P, tm, tn, smem); break;
-                default: vec ? X3TileRunner<X3_64x64, true>::run(P, tm, tn, smem)
-                             : X3TileRunner<X3_64x64, false>::run(P, tm, tn, smem); break;
+                case 0: vec ? X3TileRunner<X3_128x128, true, false>::run(P, tm, tn, smem)
+                            : X3TileRunner<X3_128x128, false, false>::run(P, tm, tn, smem); break;
+                default: vec ? X3TileRunner<X3_64x64, true, false>::run(P, tm, tn, smem)
+                             : X3TileRunner<X3_64x64, false, false>::run(P, tm, tn, smem); break;
             }
         }
     }

@@ -111,7 +111,11 @@ __device__ inline void f32_schedule() {
     }
 }
 
-template <class T, bool VEC>
+// PEEL: run the k-tiles whose successor is full through the branch-free
+// body (faster, more registers); the persistent grouped kernel, which holds
+// several tile shapes at once, turns it off to keep its register budget.
+// Both orders do the same arithmetic, so results are identical bits.
+template <class T, bool VEC, bool PEEL = true>
 struct F32TileRunner {
     static constexpr int BM = T::BM, BN = T::BN, BK = T::BK;
 
@@ -292,9 +296,11 @@ struct F32TileRunner {
         const int kfull = K - kbeg >= BK ? (K - kbeg) / BK : 0;
         const int nfast = min(ktiles, kfull) - 1;
         int kt = 0;
-        for (; kt < nfast; ++kt) {
-            iteration(kt, std::true_type{});
-            __syncthreads();
+        if constexpr (PEEL) {
+            for (; kt < nfast; ++kt) {
+                iteration(kt, std::true_type{});
+                __syncthreads();
+            }
         }
         for (; kt < ktiles; ++kt) {
             iteration(kt, std::false_type{});

@@ -137,7 +137,11 @@ __device__ inline void x3_schedule() {
     }
 }
 
-template <class T, bool VEC>
+// PEEL: run the k-tiles whose successor is full through the branch-free
+// body (faster, more registers); the persistent grouped kernel, which holds
+// several tile shapes at once, turns it off to keep its register budget.
+// Both orders do the same arithmetic, so results are identical bits.
+template <class T, bool VEC, bool PEEL = true>
 struct X3TileRunner {
     static constexpr int BM = T::BM, BN = T::BN, BK = T::BK;
 
@@ -334,9 +338,11 @@ struct X3TileRunner {
         const int kfull = K - kbeg >= BK ? (K - kbeg) / BK : 0;  // full k-tiles from kbeg
         const int nfast = min(ktiles, kfull) - 1;              // iterations with a full successor
         int kt = 0;
-        for (; kt < nfast; ++kt) {
-            iteration(kt, std::true_type{});
-            __syncthreads();
+        if constexpr (PEEL) {
+            for (; kt < nfast; ++kt) {
+                iteration(kt, std::true_type{});
+                __syncthreads();
+            }
         }
         for (; kt < ktiles; ++kt) {
             iteration(kt, std::false_type{});
