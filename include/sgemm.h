@@ -90,7 +90,12 @@ const char* sgemm_hip_error_string(void);
 /* ------------------------------------------------------------------------ */
 
 /* As sgemm(), on `stream`, returning a status.  The config is chosen by the
- * autotune table for (m, n, k, SGEMM_FORM_F32). */
+ * autotune table for (m, n, k, SGEMM_FORM_F32).  Split-K configs (chosen for
+ * grids with fewer than two workgroups per CU) use a device workspace owned
+ * by the library, one per stream, allocated on the first such call and kept
+ * until the process ends; a call on a stream under graph capture never
+ * allocates and runs unsplit if the workspace is too small.  Results of a
+ * split-K config are deterministic (partials are summed in split order). */
 int sgemm_ex(int m, int n, int k, const float* alpha, const float* A, int lda, const float* B,
              int ldb, const float* beta, float* C, int ldc, sgemm_stream_t stream);
 
@@ -102,7 +107,8 @@ int sgemm_bf16x3(int m, int n, int k, const float* alpha, const float* A, int ld
                  sgemm_stream_t stream);
 
 /* Explicit form and kernel config (config ids from sgemm_config_count /
- * sgemm_config_name).  Used by the autotuner and by tests. */
+ * sgemm_config_info; splitk configs as described at sgemm_ex).  Used by the
+ * autotuner and by tests. */
 int sgemm_run_config(int form, int config_id, int m, int n, int k, const float* alpha,
                      const float* A, int lda, const float* B, int ldb, const float* beta,
                      float* C, int ldc, sgemm_stream_t stream);
