@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--group-count", type=int, default=512)
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="arrival window per swept rate")
+    ap.add_argument("--serve-chunk-gflop", type=float, default=0.0,
+                    help="server chunk threshold in GFLOP (0 = library default, < 0 = off)")
     ap.add_argument("--serve-rates", type=str, default="0.25,0.5,0.75,0.9",
                     help="arrival rates as fractions of the measured closed-loop capacity")
     ap.add_argument("--plain-only", action="store_true",
@@ -217,7 +219,7 @@ def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0):
             "grid": info["grid"]}
 
 
-def serving_leg(sa, torch, rates, window_s, seed=0):
+def serving_leg(sa, torch, rates, window_s, chunk_gflop=0.0, seed=0):
     """Poisson arrivals of BF16x3 requests, sizes 70% [128,512], 25%
     [1024,2048], 5% [4096,8192] (each of M, N, K uniform in the bucket);
     operands resident in HBM (sub-blocks of two 8192^2 panels), 64 GB budget."""
@@ -241,7 +243,8 @@ def serving_leg(sa, torch, rates, window_s, seed=0):
         ring[0] += (m * n + 63) // 64 * 64
         return p
 
-    srv = sa.Server(64 << 30, max_batch=32, num_slots=3, form=sa.FORM_BF16X3)
+    srv = sa.Server(64 << 30, max_batch=32, num_slots=3, form=sa.FORM_BF16X3,
+                    chunk_gflop=chunk_gflop)
 
     def submit(m, n, k):
         return srv.submit(m, n, k, 1.0, Apool, side, Bpool, side, 0.0, c_slice(m, n), m)
@@ -258,7 +261,7 @@ def serving_leg(sa, torch, rates, window_s, seed=0):
     cap = srv.stats()
     capacity = cap["completed"] / cap["window_s"]
     out = {"capacity_req_s": capacity, "capacity_tflops": cap["tflops"], "budget_gb": 64,
-           "max_batch": 32, "slots": 3, "rates": []}
+           "max_batch": 32, "slots": 3, "chunk_gflop": chunk_gflop, "rates": []}
     for frac in rates:
         lam = frac * capacity
         n_req = max(50, int(lam * window_s))
@@ -267,18 +270,34 @@ def serving_leg(sa, torch, rates, window_s, seed=0):
         srv.reset_stats()
         t_start = time.perf_counter()
         t_next = t_start
+        ids = []
         for gap, s in zip(gaps, sizes):
             t_next += gap
             while time.perf_counter() < t_next:
                 pass
-            submit(*s)
+            ids.append(submit(*s))
         srv.drain(300)
         st = srv.stats()
+        # per size class: the 4096..8192 bucket ("large") against the rest
+        lat = {"small": [], "large": []}
+        for rid, (m, n, k) in zip(ids, sizes):
+            lat["large" if min(m, n, k) >= 4096 else "small"].append(srv.latency(rid)[0])
+
+        def pct(v, p):
+            if not v:
+                return None
+            v = sorted(v)
+            return v[min(len(v), max(1, int(np.ceil(p / 100 * len(v))))) - 1]
         out["rates"].append({"fraction_of_capacity": frac, "rate_req_s": lam,
                              "requests": st["measured"], "p50_ms": st["p50_ms"],
                              "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
                              "mean_queue_ms": st["mean_queue_ms"], "tflops": st["tflops"],
                              "mean_batch": st["mean_batch"],
+                             "panel_launches": st["panel_launches"],
+                             "p50_small_ms": pct(lat["small"], 50),
+                             "p99_small_ms": pct(lat["small"], 99),
+                             "p50_large_ms": pct(lat["large"], 50),
+                             "p99_large_ms": pct(lat["large"], 99),
                              "peak_gb": st["peak_bytes"] / 2 ** 30})
     srv.destroy()
     del Apool, Bpool, arena
@@ -379,7 +398,7 @@ def main():
     if not args.no_serve:
         print("bench: serving", file=sys.stderr, flush=True)
         rates = [float(x) for x in args.serve_rates.split(",") if x]
-        serving = serving_leg(sa, torch, rates, args.serve_seconds)
+        serving = serving_leg(sa, torch, rates, args.serve_seconds, args.serve_chunk_gflop)
 
     achieved = tflops
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,

@@ -189,6 +189,8 @@ int sgemm_autotune_load(const char* path);
 /* ------------------------------------------------------------------------ */
 /* Serving loop: continuous batching of arriving GEMM requests into grouped  */
 /* launches, admitted against a fixed HBM budget (no reference counterpart). */
+/* Small requests share grouped launches; large ones are cut into column    */
+/* panels that are interleaved with those launches (see chunk_gflop).       */
 /* ------------------------------------------------------------------------ */
 
 enum { SGEMM_EXECUTOR_GPU = 0, SGEMM_EXECUTOR_MOCK = 1 };
@@ -203,6 +205,10 @@ typedef struct {
                                 after flops / mock_tflops seconds) */
     double mock_tflops;
     double mock_launch_us;
+    double chunk_gflop;      /* device-operand requests above this many GFLOP (2mnk/1e9) are
+                                cut into column panels of about this size, each one launch
+                                interleaved with the small-request batches; 0 = 100 GFLOP,
+                                < 0 = never cut (every request goes into grouped batches) */
 } sgemm_server_config;
 
 /* One request.  operands == SGEMM_OPERANDS_DEVICE: A, B, C are device memory
@@ -232,6 +238,7 @@ typedef struct {
     double tflops;        /* sum of 2mnk over that window */
     size_t peak_bytes;    /* most budget bytes in use at once */
     size_t budget_bytes;
+    long long panel_launches; /* single-problem launches of large-request panels */
 } sgemm_server_stats;
 
 typedef struct sgemm_server_s* sgemm_server;

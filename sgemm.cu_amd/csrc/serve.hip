@@ -1,16 +1,31 @@
 // serve.hip -- continuous-batching GEMM server (no reference counterpart; the
 // reference has no serving layer).
 //
-// Requests arrive through sgemm_server_submit and queue in arrival order.  A
-// scheduler thread admits them against a fixed HBM budget and packs them into
-// grouped launches (grouped.hip), one launch per "slot"; every slot has its
-// own stream, so up to num_slots launches overlap on the device.  A request
-// that arrives while others are on the GPU is admitted as soon as a slot and
-// enough of the budget are free -- it does not wait for the running batches.
+// Requests arrive through sgemm_server_submit.  A scheduler thread admits
+// them against a fixed HBM budget and launches them on "slots"; every slot has
+// its own stream, so up to num_slots launches overlap on the device.  A
+// request that arrives while others are on the GPU is admitted as soon as a
+// slot and enough of the budget are free -- it does not wait for the running
+// launches.
 //
-// Admission is strict FIFO: the head request is admitted when its footprint
-// fits the free budget; later requests never overtake it, so nothing starves.
-// Footprint = 4*(m*k + k*n + m*n) bytes (the request's operands and result).
+// Two lanes, each strict FIFO:
+//   * small requests (2mnk <= chunk threshold, and every host-operand request)
+//     are packed, up to max_batch at a time, into one grouped launch
+//     (grouped.hip);
+//   * large device-operand requests are cut into column panels
+//     C[:, j0:j1] = alpha*A*B[:, j0:j1] + beta*C[:, j0:j1] of about the
+//     threshold's flops each, and every panel is one single-problem launch with
+//     the best config for its shape.  Panels of one large request go out in
+//     order; the next large request starts when the previous one has launched
+//     all of its panels.
+// A free slot alternates between the lanes (a small batch, then a panel, ...),
+// falling back to whichever lane has work.  So a small request waits for at
+// most about one panel of large work instead of a whole large request, and a
+// large request still advances by a panel every other launch: neither lane
+// starves.  A small request may overtake a large one waiting for memory only
+// if the budget still has room for that large one afterwards.
+// Footprint = 4*(m*k + k*n + m*n) bytes (the request's operands and result),
+// reserved from admission to completion.
 //   * SGEMM_OPERANDS_HOST: the server owns the device copies.  It carves them
 //     out of one pool allocated at create time (first-fit, coalescing free
 //     list), copies A, B (and C when beta != 0) in on the slot's stream, and C
@@ -18,10 +33,10 @@
 //   * SGEMM_OPERANDS_DEVICE: the operands already live on the device; the
 //     footprint is reserved against the budget while the request is in flight.
 //
-// Every request's config comes from the autotune table for its own shape, so
-// its bits do not depend on which other requests share its launch.
+// Which lane, which panels and which config a request gets depend on its own
+// shape only, so its bits do not depend on which other requests share the GPU.
 //
-// The executor is replaceable: SGEMM_EXECUTOR_MOCK completes a batch after
+// The executor is replaceable: SGEMM_EXECUTOR_MOCK completes a launch after
 // flops / mock_tflops seconds without touching a GPU, so admission, batching
 // and the statistics are testable on any host.
 #include <hip/hip_runtime.h>
@@ -106,22 +121,34 @@ struct Req {
     int status = SGEMM_STATUS_NOT_READY;
     bool pooled = false;
     size_t off[3] = {0, 0, 0};  // pool offsets of A, B, C (host operands)
+    // Large lane: column panels of panel_n columns (the last may be narrower).
+    bool large = false;
+    int panels = 0, panel_n = 0, next_panel = 0, panels_done = 0;
+    bool panel_failed = false;
 };
 
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     GroupedPlan* plan = nullptr;
-    std::vector<Req*> batch;
+    std::vector<Req*> batch;   // small lane: the grouped launch's requests
+    Req* chunk = nullptr;      // large lane: the request one panel belongs to
+    int chunk_panel = -1;
     bool busy = false;
     double mock_done_at = 0;
 };
+
+// Requests above this many flops go to the large lane (when chunking is on).
+constexpr double kDefaultChunkGflop = 100.0;
 
 }  // namespace
 
 class Server {
   public:
-    explicit Server(const sgemm_server_config& c) : cfg_(c), alloc_(c.hbm_budget_bytes) {}
+    explicit Server(const sgemm_server_config& c) : cfg_(c), alloc_(c.hbm_budget_bytes) {
+        chunk_flops_ = c.chunk_gflop < 0 ? 0.0
+                       : (c.chunk_gflop > 0 ? c.chunk_gflop : kDefaultChunkGflop) * 1e9;
+    }
 
     int start() {
         t0_ = Clock::now();
@@ -179,11 +206,22 @@ class Server {
             ++rejected_;
             return SGEMM_STATUS_BUDGET_EXCEEDED;
         }
+        // Large lane: device operands above the chunk threshold, cut into
+        // column panels of about chunk_flops_ each (multiples of 256 columns,
+        // the widest tile, unless n itself is smaller).
+        const double flops = 2.0 * r.m * r.n * (double)r.k;
+        if (chunk_flops_ > 0 && r.operands == SGEMM_OPERANDS_DEVICE && flops > chunk_flops_) {
+            long long w = (long long)(chunk_flops_ / (2.0 * r.m * (double)r.k));
+            w = std::max(256LL, w / 256 * 256);
+            q_->panel_n = (int)std::min<long long>(w, r.n);
+            q_->panels = (r.n + q_->panel_n - 1) / q_->panel_n;
+            q_->large = q_->panels > 1;
+        }
         std::lock_guard<std::mutex> lock(mu_);
         q_->id = ++next_id_;
         q_->t_submit = seconds_since(t0_);
         if (id) *id = q_->id;
-        pending_.push_back(q_.get());
+        (q_->large ? large_q_ : pending_).push_back(q_.get());
         all_[q_->id] = std::move(q_);
         ++submitted_;
         cv_.notify_all();
@@ -240,6 +278,7 @@ class Server {
         out->rejected = rejected_;
         out->failed = failed_;
         out->batches = batches_;
+        out->panel_launches = panel_launches_;
         out->mean_batch = batches_ ? (double)batched_requests_ / (double)batches_ : 0.0;
         out->peak_bytes = peak_bytes_;
         out->budget_bytes = cfg_.hbm_budget_bytes;
@@ -279,6 +318,7 @@ class Server {
         peak_bytes_ = used_bytes_;
         batches_ = 0;
         batched_requests_ = 0;
+        panel_launches_ = 0;
     }
 
   private:
@@ -314,8 +354,20 @@ class Server {
         if (e == hipErrorNotReady) return false;
         if (e != hipSuccess) {
             for (Req* q : s.batch) q->status = SGEMM_STATUS_HIP_ERROR;
+            if (s.chunk) s.chunk->status = SGEMM_STATUS_HIP_ERROR;
         }
         return true;
+    }
+
+    void finish(Req* q, double now) {
+        q->t_done = now;
+        if (q->status == SGEMM_STATUS_NOT_READY) {
+            q->status = SGEMM_STATUS_SUCCESS;
+            ++completed_;
+        } else {
+            ++failed_;
+        }
+        release(q);
     }
 
     void retire() {
@@ -323,16 +375,21 @@ class Server {
         for (Slot& s : slots_) {
             if (!s.busy || !slot_done(s)) continue;
             const double now = seconds_since(t0_);
-            for (Req* q : s.batch) {
-                q->t_done = now;
-                if (q->status == SGEMM_STATUS_NOT_READY) {
-                    q->status = SGEMM_STATUS_SUCCESS;
-                    ++completed_;
-                } else {
-                    ++failed_;
+            if (Req* q = s.chunk) {
+                ++q->panels_done;
+                if (q->status != SGEMM_STATUS_NOT_READY) q->panel_failed = true;
+                if (q->panels_done == q->panels) {
+                    if (q->panel_failed && q->status == SGEMM_STATUS_NOT_READY)
+                        q->status = SGEMM_STATUS_HIP_ERROR;
+                    finish(q, now);
                 }
-                release(q);
+                s.chunk = nullptr;
+                s.chunk_panel = -1;
+                s.busy = false;
+                any = true;
+                continue;
             }
+            for (Req* q : s.batch) finish(q, now);
             s.batch.clear();
             s.busy = false;
             any = true;
@@ -347,9 +404,10 @@ class Server {
         used_bytes_ -= q->footprint;
     }
 
-    // Reserve the head request's footprint (and pool space for host operands).
-    bool reserve(Req* q) {
-        if (used_bytes_ + q->footprint > cfg_.hbm_budget_bytes) return false;
+    // Reserve a request's footprint (and pool space for host operands),
+    // keeping `headroom` bytes of the budget free.
+    bool reserve(Req* q, size_t headroom = 0) {
+        if (used_bytes_ + q->footprint + headroom > cfg_.hbm_budget_bytes) return false;
         if (q->r.operands == SGEMM_OPERANDS_HOST) {
             const size_t sz[3] = {4 * (size_t)q->r.m * q->r.k, 4 * (size_t)q->r.k * q->r.n,
                                   4 * (size_t)q->r.m * q->r.n};
@@ -370,26 +428,88 @@ class Server {
     void admit() {
         for (Slot& s : slots_) {
             if (s.busy) continue;
-            if (pending_.empty()) return;
-            std::vector<Req*> batch;
-            while (!pending_.empty() && (int)batch.size() < cfg_.max_batch) {
-                Req* q = pending_.front();
-                if (!reserve(q)) break;  // strict FIFO: the head waits for memory
-                q->t_admit = seconds_since(t0_);
-                pending_.pop_front();
-                batch.push_back(q);
-            }
-            if (batch.empty()) return;
-            s.batch = batch;
-            s.busy = true;
-            ++batches_;
-            batched_requests_ += (long long)batch.size();
-            if (launch(s) != SGEMM_STATUS_SUCCESS) {
-                for (Req* q : s.batch) q->status = SGEMM_STATUS_HIP_ERROR;
-                s.mock_done_at = 0;  // retire at once
-                if (cfg_.executor == SGEMM_EXECUTOR_GPU) hipEventRecord(s.done, s.stream);
-            }
+            const bool large_first = prefer_large_;
+            if (!(large_first ? launch_panel(s) || launch_small(s)
+                              : launch_small(s) || launch_panel(s)))
+                return;  // nothing admissible for any slot
         }
+    }
+
+    // Small lane: a grouped launch of up to max_batch requests from the head.
+    bool launch_small(Slot& s) {
+        // A large request waiting for memory keeps its footprint free: small
+        // requests may overtake it, but never push it out of the budget.
+        const size_t headroom =
+            (!large_active_ && !large_q_.empty()) ? large_q_.front()->footprint : 0;
+        std::vector<Req*> batch;
+        while (!pending_.empty() && (int)batch.size() < cfg_.max_batch) {
+            Req* q = pending_.front();
+            if (!reserve(q, headroom)) break;  // FIFO within the lane
+            q->t_admit = seconds_since(t0_);
+            pending_.pop_front();
+            batch.push_back(q);
+        }
+        if (batch.empty()) return false;
+        s.batch = batch;
+        s.busy = true;
+        ++batches_;
+        batched_requests_ += (long long)batch.size();
+        prefer_large_ = true;
+        if (launch(s) != SGEMM_STATUS_SUCCESS) {
+            for (Req* q : s.batch) q->status = SGEMM_STATUS_HIP_ERROR;
+            s.mock_done_at = 0;  // retire at once
+            if (cfg_.executor == SGEMM_EXECUTOR_GPU) hipEventRecord(s.done, s.stream);
+        }
+        return true;
+    }
+
+    // Large lane: the next panel of the active large request, admitting the
+    // next one when the active request has launched all of its panels.
+    bool launch_panel(Slot& s) {
+        if (large_active_ && large_active_->next_panel == large_active_->panels)
+            large_active_ = nullptr;
+        if (!large_active_) {
+            if (large_q_.empty()) return false;
+            Req* q = large_q_.front();
+            if (!reserve(q)) return false;
+            q->t_admit = seconds_since(t0_);
+            large_q_.pop_front();
+            large_active_ = q;
+        }
+        Req* q = large_active_;
+        s.chunk = q;
+        s.chunk_panel = q->next_panel++;
+        s.busy = true;
+        ++panel_launches_;
+        prefer_large_ = false;
+        if (launch_chunk(s) != SGEMM_STATUS_SUCCESS) {
+            q->status = SGEMM_STATUS_HIP_ERROR;
+            s.mock_done_at = 0;
+            if (cfg_.executor == SGEMM_EXECUTOR_GPU) hipEventRecord(s.done, s.stream);
+        }
+        return true;
+    }
+
+    // One column panel as a single-problem launch (best config for its shape).
+    int launch_chunk(Slot& s) {
+        const sgemm_request& r = s.chunk->r;
+        const int j0 = s.chunk_panel * s.chunk->panel_n;
+        const int nc = std::min(s.chunk->panel_n, r.n - j0);
+        if (cfg_.executor == SGEMM_EXECUTOR_MOCK) {
+            s.mock_done_at = seconds_since(t0_) + cfg_.mock_launch_us * 1e-6 +
+                             2.0 * r.m * nc * (double)r.k / (cfg_.mock_tflops * 1e12);
+            return SGEMM_STATUS_SUCCESS;
+        }
+        const float* B = r.B + (size_t)j0 * r.ldb;
+        float* C = r.C + (size_t)j0 * r.ldc;
+        const int st = cfg_.form == SGEMM_FORM_BF16X3
+                           ? sgemm_bf16x3(r.m, nc, r.k, &r.alpha, r.A, r.lda, B, r.ldb, &r.beta,
+                                          C, r.ldc, s.stream)
+                           : sgemm_ex(r.m, nc, r.k, &r.alpha, r.A, r.lda, B, r.ldb, &r.beta, C,
+                                      r.ldc, s.stream);
+        if (st != SGEMM_STATUS_SUCCESS) return st;
+        return hipEventRecord(s.done, s.stream) == hipSuccess ? SGEMM_STATUS_SUCCESS
+                                                              : SGEMM_STATUS_HIP_ERROR;
     }
 
     int launch(Slot& s) {
@@ -452,7 +572,12 @@ class Server {
     RangeAllocator alloc_;
     void* pool_ = nullptr;
     std::vector<Slot> slots_;
-    std::deque<Req*> pending_;
+    std::deque<Req*> pending_;   // small lane, arrival order
+    std::deque<Req*> large_q_;   // large lane, arrival order
+    Req* large_active_ = nullptr;
+    bool prefer_large_ = false;
+    double chunk_flops_ = 0;
+    long long panel_launches_ = 0;
     std::unordered_map<uint64_t, std::unique_ptr<Req>> all_;
     std::mutex mu_;
     std::condition_variable cv_, done_cv_;

@@ -57,6 +57,7 @@ class ServerConfig(ctypes.Structure):
         ("hbm_budget_bytes", ctypes.c_size_t), ("max_batch", ctypes.c_int),
         ("num_slots", ctypes.c_int), ("form", ctypes.c_int), ("executor", ctypes.c_int),
         ("mock_tflops", ctypes.c_double), ("mock_launch_us", ctypes.c_double),
+        ("chunk_gflop", ctypes.c_double),
     ]
 
 
@@ -80,7 +81,7 @@ class ServerStats(ctypes.Structure):
         ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
         ("mean_queue_ms", ctypes.c_double), ("window_s", ctypes.c_double),
         ("tflops", ctypes.c_double), ("peak_bytes", ctypes.c_size_t),
-        ("budget_bytes", ctypes.c_size_t),
+        ("budget_bytes", ctypes.c_size_t), ("panel_launches", ctypes.c_longlong),
     ]
 
     def as_dict(self) -> dict:
@@ -397,9 +398,9 @@ class Server:
     """Continuous-batching GEMM server (sgemm_server_* in include/sgemm.h)."""
 
     def __init__(self, budget_bytes, max_batch=64, num_slots=2, form=FORM_BF16X3,
-                 executor=EXECUTOR_GPU, mock_tflops=0.0, mock_launch_us=0.0):
+                 executor=EXECUTOR_GPU, mock_tflops=0.0, mock_launch_us=0.0, chunk_gflop=0.0):
         cfg = ServerConfig(budget_bytes, max_batch, num_slots, form, executor, mock_tflops,
-                           mock_launch_us)
+                           mock_launch_us, chunk_gflop)
         self.handle = ctypes.c_void_p()
         _check(lib().sgemm_server_create(ctypes.byref(self.handle), ctypes.byref(cfg)),
                "sgemm_server_create")

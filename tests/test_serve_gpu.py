@@ -27,18 +27,22 @@ def _cases(count, seed):
     return out
 
 
-@pytest.mark.parametrize("form", [sa.FORM_BF16X3, sa.FORM_F32])
-def test_alone_equals_mix_and_oracle(form):
+@pytest.mark.parametrize("form,chunk_gflop", [(sa.FORM_BF16X3, 8.0), (sa.FORM_F32, 8.0),
+                                               (sa.FORM_BF16X3, -1.0)])
+def test_alone_equals_mix_and_oracle(form, chunk_gflop):
+    """chunk_gflop = 8 cuts the requests above 8 GFLOP (most of the
+    1024..3072 ones) into column panels run as single launches between the
+    grouped batches; -1 sends everything through grouped batches."""
     import torch
     cases = _cases(24, seed=7 + form)
     dev = [c.device() for c in cases]
-    srv = sa.Server(8 * GB, max_batch=8, num_slots=3, form=form)
+    srv = sa.Server(8 * GB, max_batch=8, num_slots=3, form=form, chunk_gflop=chunk_gflop)
 
     # alone: one request in the system at a time
     alone = []
     for c, (dA, dB, dC) in zip(cases, dev):
         out = upload(c.C0)
-        log(f"serve alone form {form} {(c.m, c.n, c.k)}")
+        log(f"serve alone form {form} chunk {chunk_gflop} {(c.m, c.n, c.k)}")
         rid = srv.submit(c.m, c.n, c.k, c.alpha, dA, c.lda, dB, c.ldb, c.beta, out, c.ldc)
         assert srv.wait(rid, 60) == 0
         alone.append(download(out))
@@ -46,7 +50,7 @@ def test_alone_equals_mix_and_oracle(form):
     # mix: all at once, plus a second wave while the first is in flight
     outs = [upload(c.C0) for c in cases]
     ids = []
-    log(f"serve mix form {form}")
+    log(f"serve mix form {form} chunk {chunk_gflop}")
     for i, (c, (dA, dB, _)) in enumerate(zip(cases, dev)):
         ids.append(srv.submit(c.m, c.n, c.k, c.alpha, dA, c.lda, dB, c.ldb, c.beta, outs[i],
                               c.ldc))
@@ -56,6 +60,7 @@ def test_alone_equals_mix_and_oracle(form):
     st = srv.stats()
     assert st["failed"] == 0 and st["completed"] == 2 * len(cases)
     assert st["mean_batch"] > 1.0
+    assert (st["panel_launches"] > 0) == (chunk_gflop > 0)
     tol = orc.fp32_tol_mult if form == sa.FORM_F32 else orc.bf16x3_tol_mult
     for c, a, o, rid in zip(cases, alone, outs, ids):
         assert srv.poll(rid) == 0

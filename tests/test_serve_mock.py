@@ -133,3 +133,66 @@ def test_invalid_request_rejected():
     assert srv.submit(4, 4, 4, 1.0, 4096, 4, 4096, 4, 0.0, 4096, 4, operands=7,
                       check=False) == -1
     srv.destroy()
+
+
+def test_large_request_is_cut_into_panels_and_small_ones_overtake():
+    """A request above the chunk threshold runs as column panels (one launch
+    each); small requests submitted behind it are interleaved between the
+    panels, so they finish long before it does."""
+    srv = sa.Server(256 * MB, max_batch=8, num_slots=1, executor=sa.EXECUTOR_MOCK,
+                    mock_tflops=0.01, mock_launch_us=0, chunk_gflop=0.01)
+    large = _submit(srv, 512, 1024, 512)        # 537 MFLOP -> 4 panels of 256 columns
+    time.sleep(0.002)
+    small = [_submit(srv, 64, 64, 64) for _ in range(6)]
+    srv.drain(30)
+    st = srv.stats()
+    assert st["completed"] == 7 and st["panel_launches"] == 4
+    t_large, _ = srv.latency(large)
+    for rid in small:
+        t, _ = srv.latency(rid)
+        # at most about one panel (13.4 ms of mock compute) ahead of each
+        assert t < 0.6 * t_large, (t, t_large)
+    srv.destroy()
+
+
+def test_large_request_does_not_starve_under_small_traffic():
+    """Panels alternate with small batches, so a large request keeps advancing
+    while small requests keep arriving."""
+    srv = sa.Server(256 * MB, max_batch=4, num_slots=1, executor=sa.EXECUTOR_MOCK,
+                    mock_tflops=0.01, mock_launch_us=0, chunk_gflop=0.01)
+    large = _submit(srv, 512, 1024, 512)        # 4 panels, ~13.4 ms each
+    for _ in range(60):                          # ~120 ms of steady small traffic
+        _submit(srv, 128, 128, 128)              # 0.4 ms each
+        time.sleep(0.002)
+    srv.wait(large, 30)
+    t_large, _ = srv.latency(large)
+    # 4 panels plus at most one small batch (<= 4 x 0.4 ms) before each
+    assert t_large < 4 * (13.4 + 1.6) + 40.0, t_large
+    srv.drain(30)
+    assert srv.stats()["completed"] == 61
+    srv.destroy()
+
+
+def test_small_requests_leave_room_for_a_waiting_large_one():
+    """A large request waiting for memory keeps its footprint reserved against
+    small overtakers: the small ones never push it out of the budget."""
+    # footprints: 512x1024x512 -> 4*(256K + 512K + 512K) = 5 MiB; 256^3 -> 768 KiB
+    budget = 5 * MB + MB // 2   # `first` + large does not fit; large alone does
+    srv = sa.Server(budget, max_batch=8, num_slots=3, executor=sa.EXECUTOR_MOCK,
+                    mock_tflops=0.002, mock_launch_us=0, chunk_gflop=0.01)
+    first = _submit(srv, 256, 256, 256)          # holds 768 KiB for ~17 ms
+    time.sleep(0.002)
+    large = _submit(srv, 512, 1024, 512)         # 5 MiB: waits for `first`
+    small = [_submit(srv, 256, 256, 256) for _ in range(3)]
+    srv.drain(60)
+    st = srv.stats()
+    assert st["completed"] == 5 and st["peak_bytes"] <= budget
+    _, q_large = srv.latency(large)
+    t_first, _ = srv.latency(first)
+    # the large request was admitted as soon as `first` released its memory,
+    # not after the later small requests, which could not overtake it
+    assert q_large < t_first + 5.0
+    for rid in small:
+        _, q = srv.latency(rid)
+        assert q >= q_large - 0.5
+    srv.destroy()
