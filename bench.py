@@ -164,8 +164,8 @@ def cpu_baseline(size: int, cols: int):
 def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0):
     """512 independent problems, M, N, K log-uniform integers in [32, 2048],
     U[-1, 1), alpha=1, beta=0: one persistent launch replayed from a hipGraph,
-    against the same problems as back-to-back single launches (same configs,
-    also graph-captured so no host overhead is timed)."""
+    against the same problems as back-to-back single launches of the default
+    entry point (also graph-captured, so no host overhead is timed)."""
     import numpy as np
     rng = np.random.default_rng(seed)
     shapes = np.floor(np.exp(rng.uniform(np.log(32), np.log(2049), size=(count, 3)))).astype(int)
@@ -188,26 +188,26 @@ def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0):
         if key not in tuned:
             tuned[key] = sa.autotune_bucket(form, m, n, k, iters=2, grouped=True, stream=stream)[0]
     tune_s = time.perf_counter() - t0
-    cfgs = [sa.autotune_lookup(form, m, n, k, grouped=True)[0] for m, n, k in shapes.tolist()]
     torch.cuda.synchronize()
     gs = torch.cuda.Stream()  # stream capture needs a stream other than the null stream
     plan = sa.GroupedPlan(count, form)
     plan.set(probs, gs)
     plan.capture(gs)
     ms_group = time_form(lambda: plan.replay(gs), steps, warmup, gs)
-    # the same problems and configs as single launches, captured in one graph
+    # the same problems as single launches of the library's default entry
+    # point (each with its own single-launch config), captured in one graph on
+    # a stream warmed up first (so split-K workspaces exist before capture)
+    single = sa.sgemm_ex if form == sa.FORM_F32 else sa.sgemm_bf16x3
     loop_graph = torch.cuda.CUDAGraph()
     side = torch.cuda.Stream()
     side.wait_stream(stream)
     with torch.cuda.stream(side):
-        for (m, n, k), (A, B, C), c in zip(shapes.tolist(), mats, cfgs):  # warm-up
-            sa.run_config(form, c, m, n, k, 1.0, A, m, B, k, 0.0, C, m, side)
-    stream.wait_stream(side)
-    torch.cuda.synchronize()
-    with torch.cuda.graph(loop_graph):
-        cs = torch.cuda.current_stream()
-        for (m, n, k), (A, B, C), c in zip(shapes.tolist(), mats, cfgs):
-            sa.run_config(form, c, m, n, k, 1.0, A, m, B, k, 0.0, C, m, cs)
+        for (m, n, k), (A, B, C) in zip(shapes.tolist(), mats):  # warm-up
+            single(m, n, k, 1.0, A, m, B, k, 0.0, C, m, side)
+    side.synchronize()
+    with torch.cuda.graph(loop_graph, stream=side):
+        for (m, n, k), (A, B, C) in zip(shapes.tolist(), mats):
+            single(m, n, k, 1.0, A, m, B, k, 0.0, C, m, side)
     ms_loop = time_form(loop_graph.replay, steps, warmup, stream)
     info = plan.info()
     plan.destroy()

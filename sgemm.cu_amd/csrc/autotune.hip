@@ -21,6 +21,7 @@
 
 #include "sgemm.h"
 #include "autotune.hpp"
+#include "grouped.hpp"
 #include "kernels.hpp"
 
 namespace sgemm_amd {
@@ -58,9 +59,7 @@ void unkey(int key, int& form, int& grouped, int& mb, int& nb, int& kb) {
     form = key / 2;
 }
 
-bool grouped_capable(int form, int cfg) {
-    return form == kFormF32 ? (cfg >= 0 && cfg <= 2) : (cfg >= 0 && cfg <= 1);
-}
+bool grouped_capable(int form, int cfg) { return grouped_config(form, cfg); }
 
 int load_locked(Table& t, const char* path);
 
@@ -74,7 +73,7 @@ void ensure_env_table(Table& t) {
 int heuristic(int form, int grouped, int m, int n, int k) {
     if (form == kFormF32) {
         if (grouped) {
-            if (m >= 96 && n >= 96) return 0;
+            if (m >= 96 && n >= 96) return 5;
             return m >= 96 ? 2 : 1;
         }
         // Measured on MI355X (square sweep 1024..16384 and the odd shapes):
@@ -296,14 +295,41 @@ int sgemm_autotune_bucket(int form, int grouped, int m, int n, int k, int iters,
         hipLaunchKernelGGL(k_fill_tune, dim3(1024), dim3(256), 0, s, B, nb, 2u);
         const float one = 1.f, zero = 0.f;
         const double flops = 2.0 * m * n * (double)k;
-        for (int c = 0; c < config_count(form); ++c) {
+        // Grouped buckets are timed inside a grouped launch of `copies` copies
+        // of the problem (each with its own C), enough to fill the device:
+        // a grouped launch runs many problems at once, so what matters there
+        // is a config's throughput per tile, not one problem's latency.
+        int copies = 1;
+        GroupedPlan* plan = nullptr;
+        std::vector<sgemm_problem> probs;
+        float* Cc = nullptr;
+        if (grouped) {
+            const long long t128 = (long long)div_up(m, 128) * div_up(n, 128);
+            copies = (int)std::min<long long>(256, div_up(4LL * device_cu_count(), t128));
+            while (copies > 1 && (size_t)copies * nc * 4 > ((size_t)512 << 20)) copies /= 2;
+            if (hipMalloc(&Cc, (size_t)copies * nc * 4) != hipSuccess ||
+                GroupedPlan::create(&plan, copies, form) != SGEMM_STATUS_SUCCESS) {
+                status = SGEMM_STATUS_ALLOC_FAILED;
+                copies = 0;
+            }
+            for (int i = 0; i < copies; ++i)
+                probs.push_back(sgemm_problem{m, n, k, 1.f, 0.f, A, m, B, k, Cc + (size_t)i * nc, m});
+        }
+        for (int c = 0; copies > 0 && c < config_count(form); ++c) {
             if (grouped && !grouped_capable(form, c)) continue;
+            auto once = [&]() -> int {
+                if (!grouped) return sgemm_run_config(form, c, m, n, k, &one, A, m, B, k, &zero, C, m, s);
+                return plan->run(s);
+            };
+            if (grouped) {
+                std::vector<int> cfgs(copies, c);
+                if (plan->set(copies, probs.data(), s, cfgs.data()) != SGEMM_STATUS_SUCCESS) continue;
+            }
             // warm-up, then the timed repetitions
-            int st = sgemm_run_config(form, c, m, n, k, &one, A, m, B, k, &zero, C, m, s);
+            int st = once();
             if (st != SGEMM_STATUS_SUCCESS) continue;
             hipEventRecord(e0, s);
-            for (int r = 0; r < iters; ++r)
-                sgemm_run_config(form, c, m, n, k, &one, A, m, B, k, &zero, C, m, s);
+            for (int r = 0; r < iters; ++r) once();
             hipEventRecord(e1, s);
             if (hipEventSynchronize(e1) != hipSuccess) {
                 status = SGEMM_STATUS_HIP_ERROR;
@@ -311,12 +337,18 @@ int sgemm_autotune_bucket(int form, int grouped, int m, int n, int k, int iters,
             }
             float ms = 0.f;
             hipEventElapsedTime(&ms, e0, e1);
-            const float tf = (float)(flops * iters / (ms * 1e-3) / 1e12);
+            const float tf = (float)(flops * copies * iters / (ms * 1e-3) / 1e12);
             if (best < 0 || tf > best_tf) {
                 best = c;
                 best_tf = tf;
             }
         }
+        if (plan) {
+            hipStreamSynchronize(s);
+            plan->release();
+            delete plan;
+        }
+        if (Cc) hipFree(Cc);
     }
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);

@@ -44,19 +44,19 @@ struct GroupHeader {
 };
 static_assert(sizeof(GroupHeader) == 64, "header size");
 
-// Configs a grouped kernel carries (all 256 threads).  The kernel's registers
-// and LDS are the maximum over the configs it contains, so the widest
-// single-problem config (f32 256x128) is left out.
-constexpr bool grouped_has(int form, int cfg) {
-    return form == kFormF32 ? (cfg >= 0 && cfg <= 2) : (cfg >= 0 && cfg <= 1);
-}
+// Configs a grouped kernel carries (grouped_config, all 256 threads).  The
+// kernel's registers and LDS are the maximum over them: for FP32 that is
+// 128x64x32's 52 KB of LDS and 128x128x16's registers, three workgroups per
+// CU -- the 128x128x32 tile (70 KB, two per CU) is left out.
+constexpr bool grouped_has(int form, int cfg) { return grouped_config(form, cfg); }
 
-constexpr int kGroupedLdsF32 = F32_128x128::LDS_FLOATS * 4;
+constexpr int kGroupedLdsF32 = F32_128x64::LDS_FLOATS * 4;
 constexpr int kGroupedLdsX3 = X3_128x128::LDS_BYTES;
 static_assert(F32_64x64::LDS_FLOATS * 4 <= kGroupedLdsF32 &&
-                  F32_128x64::LDS_FLOATS * 4 <= kGroupedLdsF32,
+                  F32_128x128_k16::LDS_FLOATS * 4 <= kGroupedLdsF32,
               "F32 grouped LDS");
 static_assert(X3_64x64::LDS_BYTES <= kGroupedLdsX3, "X3 grouped LDS");
+constexpr int kGroupedWpeF32 = 3;
 
 template <class T, bool VEC>
 __device__ inline void run_tile_f32(const Problem& P, int tm, int tn, char* smem) {
@@ -64,8 +64,9 @@ __device__ inline void run_tile_f32(const Problem& P, int tm, int tn, char* smem
 }
 
 template <int FORM>
-__global__ __launch_bounds__(256) void k_grouped(const GroupHeader* __restrict__ hdr,
-                                                 const Problem* __restrict__ probs) {
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(FORM == kFormF32 ? kGroupedWpeF32 : 1)))
+void k_grouped(const GroupHeader* __restrict__ hdr, const Problem* __restrict__ probs) {
     __shared__ __attribute__((aligned(16))) char smem[FORM == kFormF32 ? kGroupedLdsF32
                                                                         : kGroupedLdsX3];
     const int count = hdr->count;
@@ -88,8 +89,8 @@ __global__ __launch_bounds__(256) void k_grouped(const GroupHeader* __restrict__
         const bool vec = (P.flags & kProblemVec4) != 0;
         if constexpr (FORM == kFormF32) {
             switch (P.cfg) {
-                case 0: vec ? run_tile_f32<F32_128x128, true>(P, tm, tn, smem)
-                            : run_tile_f32<F32_128x128, false>(P, tm, tn, smem); break;
+                case 5: vec ? run_tile_f32<F32_128x128_k16, true>(P, tm, tn, smem)
+                            : run_tile_f32<F32_128x128_k16, false>(P, tm, tn, smem); break;
                 case 1: vec ? run_tile_f32<F32_64x64, true>(P, tm, tn, smem)
                             : run_tile_f32<F32_64x64, false>(P, tm, tn, smem); break;
                 default: vec ? run_tile_f32<F32_128x64, true>(P, tm, tn, smem)

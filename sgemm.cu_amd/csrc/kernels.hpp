@@ -33,6 +33,13 @@ hipError_t splitk_plan(hipStream_t stream, long long tiles, int k, int bk, size_
 hipError_t splitk_workspace(hipStream_t stream, size_t partial_bytes, int tiles,
                             float** partials, int** counters);
 
+// Configs the persistent grouped kernel carries (grouped.hip).  Its
+// registers and LDS are the maximum over these, which sets its occupancy:
+// FP32 {1, 2, 5} fit three workgroups per CU, BF16x3 {0, 1} one.
+constexpr bool grouped_config(int form, int cfg) {
+    return form == kFormF32 ? (cfg == 1 || cfg == 2 || cfg == 5) : (cfg == 0 || cfg == 1);
+}
+
 // Plain FP32 configs (sgemm_f32.hip).
 int f32_config_count();
 const ConfigInfo& f32_config(int id);

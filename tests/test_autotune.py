@@ -29,7 +29,7 @@ def test_default_heuristic_is_deterministic_and_valid():
             assert a == sa.autotune_lookup(form, *shape)
             assert 0 <= a[0] < n_cfg and a[1] is False
             g = sa.autotune_lookup(form, *shape, grouped=True)
-            assert g[1] is False and g[0] in ((0, 1, 2) if form == sa.FORM_F32 else (0, 1))
+            assert g[1] is False and g[0] in ((1, 2, 5) if form == sa.FORM_F32 else (0, 1))
 
 
 def test_set_applies_to_whole_bucket_only():
