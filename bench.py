@@ -155,10 +155,21 @@ def cpu_baseline(size: int, cols: int):
     orc.sgemm_basic_cm(m, cols, k, 1.5, A, m, B, k, 0.5, C, m)
     dt = time.perf_counter() - t0
     flops = 2.0 * m * cols * k
+    # the 1024^3 case as well (median of 5), as a second reference point
+    s = 1024
+    A1, B1 = orc.fill("uniform", s * s, 1), orc.fill("uniform", s * s, 2)
+    times = []
+    for _ in range(5):
+        C1 = orc.fill("uniform", s * s, 3)
+        t1 = time.perf_counter()
+        orc.sgemm_basic_cm(s, s, s, 1.5, A1, s, B1, s, 0.5, C1, s)
+        times.append(time.perf_counter() - t1)
+    t1024 = float(np.median(times))
     return {"value": flops / dt / 1e12, "unit": "TFLOPS", "cores": orc.max_threads(),
             "kind": "port",
             "sample": f"{cols} of {size} output columns of the {size}^3 problem "
-                      f"(M={m}, N={cols}, K={k}), alpha=1.5 beta=0.5, {dt:.1f} s"}
+                      f"(M={m}, N={cols}, K={k}), alpha=1.5 beta=0.5, {dt:.1f} s",
+            "tflops_1024": 2.0 * s ** 3 / t1024 / 1e12, "ms_1024_median_of_5": t1024 * 1e3}
 
 
 def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0):
