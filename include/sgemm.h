@@ -56,8 +56,10 @@ enum {
  * asynchronous.  A is m x k (lda >= max(1,m)), B is k x n (ldb >= max(1,k)),
  * C is m x n (ldc >= max(1,m)).  beta == 0 means C is not read.
  * m == 0 or n == 0 is a no-op; k == 0 gives C := beta*C.  Invalid arguments
- * launch nothing and are reported by sgemm_get_last_error().  lda and ldb
- * above 2^21 are not supported (SGEMM_STATUS_NOT_SUPPORTED).
+ * launch nothing and are reported by sgemm_get_last_error().  lda or ldb
+ * above 2^21 (past the tiled kernels' 32-bit buffer offsets) run a simple
+ * FP32 kernel with 64-bit addressing -- correct, not fast; sgemm_run_config
+ * rejects them (SGEMM_STATUS_NOT_SUPPORTED).
  * Replaces sgemm() of src/sgemm.cuh:19-48 (which is row-major and ignores
  * ldb/ldc in its fast kernels; this one honours every leading dimension). */
 void sgemm(int m, int n, int k, const float* alpha, float* A, int lda, float* B, int ldb,

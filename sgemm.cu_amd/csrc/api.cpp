@@ -72,10 +72,16 @@ int run(int form, int config_id, int m, int n, int k, const float* alpha, const 
     if (form != SGEMM_FORM_F32 && form != SGEMM_FORM_BF16X3) return SGEMM_STATUS_NOT_SUPPORTED;
     if (config_id >= config_count(form)) return SGEMM_STATUS_NOT_SUPPORTED;
     if (m == 0 || n == 0) return SGEMM_STATUS_SUCCESS;
-    if (lda > kMaxLd || ldb > kMaxLd) return SGEMM_STATUS_NOT_SUPPORTED;
     Problem p = make_problem(m, n, k, *alpha, A, lda, B, ldb, *beta, C, ldc);
     // No product term: C := beta*C (BLAS: A and B are not referenced).
     if (k == 0 || *alpha == 0.f) return hip_status(launch_scale_c(p, stream));
+    // Leading dimensions past the tiles' 32-bit buffer offsets: the default
+    // selection falls back to the naive FP32 kernel (64-bit addressing); an
+    // explicit config cannot run such a problem.
+    if (lda > kMaxLd || ldb > kMaxLd) {
+        if (config_id >= 0) return SGEMM_STATUS_NOT_SUPPORTED;
+        return hip_status(launch_wide(p, stream));
+    }
     if (config_id < 0) config_id = choose_config(form, m, n, k);
     hipError_t e = form == SGEMM_FORM_F32 ? launch_f32(config_id, p, stream)
                                           : launch_bf16x3(config_id, p, stream);

@@ -52,6 +52,8 @@ hipError_t launch_bf16x3(int id, const Problem& p, hipStream_t stream);
 
 // Naive one-thread-per-output kernel (sgemm_f32.hip).
 hipError_t launch_basic(const Problem& p, hipStream_t stream);
+// The same with BLAS beta semantics, for leading dimensions above kMaxLd.
+hipError_t launch_wide(const Problem& p, hipStream_t stream);
 
 // C := beta*C for the k == 0 / alpha == 0 shortcut (sgemm_f32.hip).
 hipError_t launch_scale_c(const Problem& p, hipStream_t stream);

@@ -80,8 +80,12 @@ hipError_t launch_f32(int id, const Problem& p, hipStream_t s) {
 // ------------------------------------------------------------ naive kernel
 // Column-major restatement of kernel_basic (src/kernels/basic.cuh:15-25): one
 // thread per output, sequential k with FMA, *alpha, then fma(C, beta, acc).
-// C is always read, as in the reference.
-__global__ __launch_bounds__(256) void k_sgemm_basic(Problem P) {
+// C is always read, as in the reference.  WIDE: the fallback for leading
+// dimensions the tiles' 32-bit buffer offsets cannot reach -- 64-bit
+// addressing, FP32 arithmetic in either form, and BLAS beta semantics
+// (beta == 0 stores without reading C).
+template <bool WIDE>
+__global__ __launch_bounds__(256) void k_sgemm_naive(Problem P) {
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (i >= P.m || j >= P.n) return;
@@ -91,17 +95,27 @@ __global__ __launch_bounds__(256) void k_sgemm_basic(Problem P) {
     for (int p = 0; p < P.k; ++p) acc = __builtin_fmaf(a[(size_t)p * P.lda], b[p], acc);
     acc *= P.alpha;
     float* c = P.C + i + (size_t)j * P.ldc;
-    *c = __builtin_fmaf(*c, P.beta, acc);
+    *c = (WIDE && P.beta == 0.f) ? acc : __builtin_fmaf(*c, P.beta, acc);
 }
 
-hipError_t launch_basic(const Problem& p, hipStream_t s) {
+template <bool WIDE>
+static hipError_t launch_naive(const Problem& p, hipStream_t s) {
     if (p.m <= 0 || p.n <= 0) return hipSuccess;
-    dim3 grid(div_up(p.m, 64), div_up(p.n, 4));
-    if (grid.y > 65535) return hipErrorInvalidValue;
     (void)hipGetLastError();
-    hipLaunchKernelGGL(k_sgemm_basic, grid, dim3(256), 0, s, p);
+    // Chunk the column range to stay within the grid's y limit.
+    for (long long j0 = 0; j0 < p.n; j0 += 4LL * 65535) {
+        Problem q = p;
+        q.n = (int)std::min<long long>(p.n - j0, 4LL * 65535);
+        q.B = p.B + (size_t)j0 * p.ldb;
+        q.C = p.C + (size_t)j0 * p.ldc;
+        hipLaunchKernelGGL(k_sgemm_naive<WIDE>, dim3(div_up(q.m, 64), div_up(q.n, 4)), dim3(256), 0,
+                           s, q);
+    }
     return hipGetLastError();
 }
+
+hipError_t launch_basic(const Problem& p, hipStream_t s) { return launch_naive<false>(p, s); }
+hipError_t launch_wide(const Problem& p, hipStream_t s) { return launch_naive<true>(p, s); }
 
 // --------------------------------------------------------- C := beta * C
 // Used when k == 0 or alpha == 0 (no product term).  beta == 0 stores zeros

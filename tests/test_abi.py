@@ -68,11 +68,15 @@ def test_config_queries(lib):
     assert "gfx950" in sa.version()
 
 
-def test_huge_leading_dimension_not_supported(lib):
+def test_huge_leading_dimension_explicit_config_not_supported(lib):
+    """An explicit tile config cannot address leading dimensions above 2^21
+    (its buffer offsets are 32-bit): rejected on the host, nothing launched.
+    The default selection runs such problems on the wide fallback kernel
+    (tests/test_sgemm_gpu.py::test_huge_leading_dimension_fallback)."""
     p = 4096
     big = (1 << 21) + 1
-    assert sa.sgemm_ex(4, 4, 4, 1.0, p, big, p, 4, 0.0, p, 4, check=False) == 2
-    assert sa.sgemm_ex(4, 4, 4, 1.0, p, 4, p, big, 0.0, p, 4, check=False) == 2
+    assert sa.run_config(0, 0, 4, 4, 4, 1.0, p, big, p, 4, 0.0, p, 4, check=False) == 2
+    assert sa.run_config(1, 2, 4, 4, 4, 1.0, p, 4, p, big, 0.0, p, 4, check=False) == 2
 
 
 def test_unknown_config_rejected(lib):

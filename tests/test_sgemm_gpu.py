@@ -320,3 +320,48 @@ def test_splitk_capture_without_warmup_runs_unsplit():
         g.replay()
         torch.cuda.synchronize()
         assert np.array_equal(download(dC), unsplit), rep
+
+
+@pytest.mark.parametrize("form", [sa.FORM_F32, sa.FORM_BF16X3])
+def test_result_larger_than_4gib(form):
+    """C of 65539 x 16389 floats (4.3 GB): element offsets pass 2^32 bytes, so
+    the epilogue's 64-bit addressing and the per-k-tile buffer rebasing are
+    exercised; beta = 0 over a NaN-filled C also checks that C is not read.
+    Sampled entries (corners included) against FP64."""
+    import torch
+    m, n, k = 65539, 16389, 67
+    lda, ldb, ldc = m + 1, k, m
+    A = orc.fill("uniform", orc.colmajor_size(m, k, lda), 901)
+    B = orc.fill("uniform", orc.colmajor_size(k, n, ldb), 902)
+    dA, dB = upload(A), upload(B)
+    dC = torch.full((ldc * n,), float("nan"), device="cuda", dtype=torch.float32)
+    log(f"4GiB C form {form}")
+    fn = sa.sgemm_ex if form == sa.FORM_F32 else sa.sgemm_bf16x3
+    fn(m, n, k, 1.0, dA, lda, dB, ldb, 0.0, dC, ldc)
+    rng = np.random.default_rng(5)
+    ii = np.concatenate([rng.integers(0, m, 4000), [0, m - 1, 0, m - 1]]).astype(np.int64)
+    jj = np.concatenate([rng.integers(0, n, 4000), [0, 0, n - 1, n - 1]]).astype(np.int64)
+    idx = torch.from_numpy(ii + jj * ldc).to("cuda")
+    got = dC[idx].double().cpu().numpy()
+    ref, scale = orc.dgemm_entries(k, 1.0, A, lda, B, ldb, 0.0, None, ldc, ii, jj)
+    tol = (orc.fp32_tol_mult(k) if form == sa.FORM_F32 else orc.bf16x3_tol_mult(k)) * scale
+    assert np.all(np.isfinite(got))
+    assert np.all(np.abs(got - ref) <= tol), float(np.max(np.abs(got - ref) / tol))
+    del dC
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("form", [sa.FORM_F32, sa.FORM_BF16X3])
+def test_huge_leading_dimension_fallback(form):
+    """lda or ldb above 2^21 (past the tiles' 32-bit buffer offsets): the
+    default selection runs the wide fallback (naive FP32, 64-bit addressing,
+    beta = 0 does not read C); results within the FP32 tolerance."""
+    big = (1 << 21) + 3
+    fn = sa.sgemm_ex if form == sa.FORM_F32 else sa.sgemm_bf16x3
+    for (m, n, k, lda, ldb) in [(40, 3, 5, big, 5), (7, 3, 33, 7, big)]:
+        c = Case(m, n, k, lda=lda, ldb=ldb, seed=m + k, alpha=1.5, beta=0.0, c_fill=np.nan)
+        dA, dB, dC = c.device()
+        log(f"wide ld form {form} {(m, n, k, lda, ldb)}")
+        fn(m, n, k, c.alpha, dA, c.lda, dB, c.ldb, c.beta, dC, c.ldc)
+        out = download(dC)
+        assert_ok(check(c, out, orc.fp32_tol_mult(k)), f"wide ld {(m, n, k, lda, ldb)}")
