@@ -365,3 +365,45 @@ def test_huge_leading_dimension_fallback(form):
         fn(m, n, k, c.alpha, dA, c.lda, dB, c.ldb, c.beta, dC, c.ldc)
         out = download(dC)
         assert_ok(check(c, out, orc.fp32_tol_mult(k)), f"wide ld {(m, n, k, lda, ldb)}")
+
+
+def test_concurrent_host_threads_on_separate_streams():
+    """Entry points are reentrant: four host threads, each on its own stream,
+    run split-K and tiled problems at once (each stream gets its own split-K
+    workspace); every result equals the one-thread result bit for bit."""
+    import threading
+    import torch
+    shapes = [(1024, 1024, 1024), (256, 256, 8192), (8192, 64, 4096), (3000, 500, 700)]
+    cases = [Case(m, n, k, seed=11 + i, alpha=1.0, beta=0.5) for i, (m, n, k) in enumerate(shapes)]
+    dev = [c.device() for c in cases]
+    ref = []
+    for c, (dA, dB, dC) in zip(cases, dev):
+        out = dC.clone()
+        sa.sgemm_ex(c.m, c.n, c.k, c.alpha, dA, c.lda, dB, c.ldb, c.beta, out, c.ldc)
+        ref.append(download(out))
+    outs = [[None] * 5 for _ in cases]
+    errors = []
+
+    def worker(i):
+        try:
+            c, (dA, dB, dC) = cases[i], dev[i]
+            s = torch.cuda.Stream()
+            for r in range(5):
+                out = dC.clone()
+                torch.cuda.synchronize()
+                sa.sgemm_ex(c.m, c.n, c.k, c.alpha, dA, c.lda, dB, c.ldb, c.beta, out, c.ldc, s)
+                s.synchronize()
+                outs[i][r] = out.cpu().numpy()
+        except Exception as e:  # reported below, in the main thread
+            errors.append(repr(e))
+
+    log("concurrent threads")
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(len(cases))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(120)
+    assert not errors, errors
+    for i in range(len(cases)):
+        for r in range(5):
+            assert np.array_equal(outs[i][r], ref[i]), (shapes[i], r)
