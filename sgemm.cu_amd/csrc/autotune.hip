@@ -97,27 +97,37 @@ int heuristic(int form, int grouped, int m, int n, int k) {
         }
         return 7;
     }
-    if (grouped) return (m >= 96 && n >= 96) ? 0 : 1;
-    // BF16x3: all configs but 5 (256x128, two per CU) and 7 (64x64 with
-    // split-K, two per CU) hold one workgroup per CU, and every tile of a
-    // config takes the same time, so the launch takes ceil(workgroups /
-    // slots) rounds.  Cost per config = rounds * (tile area per CU) / splits /
-    // (relative rate measured at 8192^3); the cheapest wins.  Config 7 splits
-    // k when its grid is short of two workgroups per CU (splitk_count).
+    if (grouped) return (m >= 96 && n >= 96) ? 8 : 1;
+    // BF16x3: configs 2 and 3 hold one workgroup per CU, 5, 7 (64x64 with
+    // split-K) and 8 (128x128) two.  Every tile of a config takes the same
+    // time, so a launch runs in rounds of CUs x per_cu workgroups.  A full
+    // round costs per_cu x (tile area / splits) / rate, rate being the
+    // config's relative speed measured at 8192^3.  In a last, partial round
+    // a CU holds w < per_cu workgroups, which then run faster each: a lone
+    // one of a two-per-CU config gets about 60 % of the CU (measured at
+    // 2048^3 with config 8).  The cheapest config wins.  Config 7 splits k
+    // when its grid is short of two workgroups per CU (splitk_count).
     struct Cand { int id, bm, bn, bk, per_cu; double rate; bool split; };
     static const Cand cands[] = {
         {2, 256, 256, 32, 1, 1.00, false}, {5, 256, 128, 16, 2, 0.92, false},
-        {3, 256, 128, 32, 1, 0.69, false}, {0, 128, 128, 32, 1, 0.50, false},
+        {8, 128, 128, 32, 2, 0.84, false}, {3, 256, 128, 32, 1, 0.69, false},
         {7, 64, 64, 64, 2, 0.47, true},
     };
+    const long long cus = device_cu_count();
     int best = 7;
     double best_cost = 0;
     for (const Cand& c : cands) {
         const long long tiles = (long long)div_up(m, c.bm) * div_up(n, c.bn);
         const long long splits = c.split ? splitk_count(tiles, k, c.bk) : 1;
-        const long long slots = (long long)device_cu_count() * c.per_cu;
-        const double cost =
-            (double)div_up(tiles * splits, slots) * c.per_cu * c.bm * c.bn / c.rate / splits;
+        const long long wgs = tiles * splits, slots = cus * c.per_cu;
+        const double work = (double)c.bm * c.bn / splits / c.rate;  // one workgroup, full CU share
+        double cost = (double)(wgs / slots) * c.per_cu * work;
+        const long long rem = wgs % slots;
+        if (rem > 0) {
+            const long long w = div_up(rem, cus);
+            const double u = w >= c.per_cu ? 1.0 : 0.6 + 0.4 * (double)(w - 1) / (c.per_cu - 1);
+            cost += (double)w * work / u;
+        }
         if (best_cost == 0 || cost < best_cost) {
             best = c.id;
             best_cost = cost;

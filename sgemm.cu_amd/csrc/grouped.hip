@@ -47,16 +47,18 @@ static_assert(sizeof(GroupHeader) == 64, "header size");
 // Configs a grouped kernel carries (grouped_config, all 256 threads).  The
 // kernel's registers and LDS are the maximum over them: for FP32 that is
 // 128x64x32's 52 KB of LDS and 128x128x16's registers, three workgroups per
-// CU -- the 128x128x32 tile (70 KB, two per CU) is left out.
+// CU -- the 128x128x32 tile (70 KB, two per CU) is left out; for BF16x3 the
+// single-accumulator 128x128 tile's 80 KB, two per CU.
 constexpr bool grouped_has(int form, int cfg) { return grouped_config(form, cfg); }
 
 constexpr int kGroupedLdsF32 = F32_128x64::LDS_FLOATS * 4;
-constexpr int kGroupedLdsX3 = X3_128x128::LDS_BYTES;
+constexpr int kGroupedLdsX3 = X3_128x128_2cu::LDS_BYTES;
 static_assert(F32_64x64::LDS_FLOATS * 4 <= kGroupedLdsF32 &&
                   F32_128x128_k16::LDS_FLOATS * 4 <= kGroupedLdsF32,
               "F32 grouped LDS");
 static_assert(X3_64x64::LDS_BYTES <= kGroupedLdsX3, "X3 grouped LDS");
 constexpr int kGroupedWpeF32 = 3;
+constexpr int kGroupedWpeX3 = 2;
 
 template <class T, bool VEC>
 __device__ inline void run_tile_f32(const Problem& P, int tm, int tn, char* smem) {
@@ -65,7 +67,7 @@ __device__ inline void run_tile_f32(const Problem& P, int tm, int tn, char* smem
 
 template <int FORM>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(FORM == kFormF32 ? kGroupedWpeF32 : 1)))
+__attribute__((amdgpu_waves_per_eu(FORM == kFormF32 ? kGroupedWpeF32 : kGroupedWpeX3)))
 void k_grouped(const GroupHeader* __restrict__ hdr, const Problem* __restrict__ probs) {
     __shared__ __attribute__((aligned(16))) char smem[FORM == kFormF32 ? kGroupedLdsF32
                                                                         : kGroupedLdsX3];
@@ -98,8 +100,8 @@ void k_grouped(const GroupHeader* __restrict__ hdr, const Problem* __restrict__ 
             }
         } else {
             switch (P.cfg) {
-                case 0: vec ? X3TileRunner<X3_128x128, true, false>::run(P, tm, tn, smem)
-                            : X3TileRunner<X3_128x128, false, false>::run(P, tm, tn, smem); break;
+                case 8: vec ? X3TileRunner<X3_128x128_2cu, true, false>::run(P, tm, tn, smem)
+                            : X3TileRunner<X3_128x128_2cu, false, false>::run(P, tm, tn, smem); break;
                 default: vec ? X3TileRunner<X3_64x64, true, false>::run(P, tm, tn, smem)
                              : X3TileRunner<X3_64x64, false, false>::run(P, tm, tn, smem); break;
             }

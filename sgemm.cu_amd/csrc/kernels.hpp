@@ -35,9 +35,9 @@ hipError_t splitk_workspace(hipStream_t stream, size_t partial_bytes, int tiles,
 
 // Configs the persistent grouped kernel carries (grouped.hip).  Its
 // registers and LDS are the maximum over these, which sets its occupancy:
-// FP32 {1, 2, 5} fit three workgroups per CU, BF16x3 {0, 1} one.
+// FP32 {1, 2, 5} fit three workgroups per CU, BF16x3 {1, 8} two.
 constexpr bool grouped_config(int form, int cfg) {
-    return form == kFormF32 ? (cfg == 1 || cfg == 2 || cfg == 5) : (cfg == 0 || cfg == 1);
+    return form == kFormF32 ? (cfg == 1 || cfg == 2 || cfg == 5) : (cfg == 1 || cfg == 8);
 }
 
 // Plain FP32 configs (sgemm_f32.hip).

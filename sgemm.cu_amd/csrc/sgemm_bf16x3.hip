@@ -20,7 +20,8 @@ void k_sgemm_bf16x3(Problem P, SplitK sk) {
     X3TileRunner<T, VEC>::run(P, tm, tn, smem, SPLIT ? &sk : nullptr, split);
 }
 
-// Ids 6-7 run the tiles of ids 0-1 with split-K (splitk = 1).
+// Ids 6-7 run the tiles of ids 0-1 with split-K (splitk = 1); id 8 is a
+// 128x128 tile with one accumulator set, two workgroups per CU.
 static const ConfigInfo kBf16x3Configs[] = {
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m16_dual"},
     {64, 64, 64, 2, 2, "bf16x3_64x64x64_w2x2_m16_dual"},
@@ -30,6 +31,7 @@ static const ConfigInfo kBf16x3Configs[] = {
     {256, 128, 16, 2, 2, "bf16x3_256x128x16_w2x2_m32_single_mid"},
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m16_dual_splitk", 1},
     {64, 64, 64, 2, 2, "bf16x3_64x64x64_w2x2_m16_dual_splitk", 1},
+    {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m32_single_2cu"},
 };
 
 int bf16x3_config_count() { return (int)(sizeof(kBf16x3Configs) / sizeof(kBf16x3Configs[0])); }

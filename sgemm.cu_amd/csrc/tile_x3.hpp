@@ -40,9 +40,11 @@ typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 // right after the first k-step's MFMAs instead of after the last one; it pays
 // when two independent workgroups share a CU and hurts a lone wave per SIMD,
 // whose loads then have less time to land.  SCHED applies x3_schedule's
-// interleaving hints to the full k-tiles.
+// interleaving hints to the full k-tiles.  WPE_ > 0 caps the registers so
+// that many waves per SIMD fit (two 4-wave workgroups per CU: their barriers
+// fall at different times, so one's MFMAs fill the other's bubbles).
 template <int BM_, int BN_, int BK_, int WM_, int WN_, int MF_ = 16, bool DUAL_ = true,
-          bool MID_ = false, bool SCHED_ = false>
+          bool MID_ = false, bool SCHED_ = false, int WPE_ = 0>
 struct X3Tile {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
     static constexpr int MF = MF_;
@@ -50,8 +52,9 @@ struct X3Tile {
     static constexpr bool MID = MID_;
     static constexpr bool SCHED = SCHED_;
     // Waves per SIMD the kernel's registers must allow: two independent
-    // workgroups per CU for the MID configs, else whatever the tile needs.
-    static constexpr int WPE = MID_ ? 2 : 1;
+    // workgroups per CU for the MID configs or when WPE_ asks for it, else
+    // whatever the tile needs.
+    static constexpr int WPE = WPE_ > 0 ? WPE_ : (MID_ ? 2 : 1);
     static constexpr int NT = 64 * WM * WN;
     static constexpr int TM = BM / WM, TN = BN / WN;
     static constexpr int MB = TM / MF, NB = TN / MF;     // MFMA blocks per wave
@@ -469,6 +472,7 @@ using X3_256x256_w8 = X3Tile<256, 256, 32, 2, 4, 32, false, false, true>;
 using X3_256x128_w4 = X3Tile<256, 128, 32, 2, 2, 32, false>;
 using X3_128x128_m32 = X3Tile<128, 128, 32, 2, 2, 32, true>;
 using X3_256x128_k16 = X3Tile<256, 128, 16, 2, 2, 32, false, true>;
+using X3_128x128_2cu = X3Tile<128, 128, 32, 2, 2, 32, false, false, false, 2>;
 
 template <class F>
 __host__ __device__ inline bool dispatch_x3(int id, F&& f) {
@@ -479,6 +483,7 @@ __host__ __device__ inline bool dispatch_x3(int id, F&& f) {
         case 3: f(TypeTag<X3_256x128_w4>{}); return true;
         case 4: f(TypeTag<X3_128x128_m32>{}); return true;
         case 5: f(TypeTag<X3_256x128_k16>{}); return true;
+        case 8: f(TypeTag<X3_128x128_2cu>{}); return true;
         default: return false;
     }
 }

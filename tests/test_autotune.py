@@ -29,7 +29,7 @@ def test_default_heuristic_is_deterministic_and_valid():
             assert a == sa.autotune_lookup(form, *shape)
             assert 0 <= a[0] < n_cfg and a[1] is False
             g = sa.autotune_lookup(form, *shape, grouped=True)
-            assert g[1] is False and g[0] in ((1, 2, 5) if form == sa.FORM_F32 else (0, 1))
+            assert g[1] is False and g[0] in ((1, 2, 5) if form == sa.FORM_F32 else (1, 8))
 
 
 def test_set_applies_to_whole_bucket_only():
@@ -51,7 +51,7 @@ def test_invalid_entries_rejected():
 
 def test_save_load_roundtrip_and_bad_file():
     sa.autotune_set(sa.FORM_F32, 64, 64, 64, 1)
-    sa.autotune_set(sa.FORM_BF16X3, 4096, 4096, 4096, 0, grouped=True)
+    sa.autotune_set(sa.FORM_BF16X3, 4096, 4096, 4096, 8, grouped=True)
     with tempfile.TemporaryDirectory() as d:
         path = os.path.join(d, "t.json")
         sa.autotune_save(path)
@@ -63,7 +63,7 @@ def test_save_load_roundtrip_and_bad_file():
         sa.autotune_load(path)
         assert sa.autotune_size() == 2
         assert sa.autotune_lookup(sa.FORM_F32, 60, 50, 40) == (1, True)
-        assert sa.autotune_lookup(sa.FORM_BF16X3, 4000, 3000, 2100, grouped=True) == (0, True)
+        assert sa.autotune_lookup(sa.FORM_BF16X3, 4000, 3000, 2100, grouped=True) == (8, True)
         bad = os.path.join(d, "bad.json")
         with open(bad, "w") as f:
             f.write('{"entries": [\n  {"form": 9, "grouped": 0}\n]}\n')

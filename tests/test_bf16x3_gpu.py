@@ -21,7 +21,7 @@ from gpu_util import Case, assert_ok, check, compact, download, log, run_form, u
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gemm_cases.json")
-N_X3_CONFIGS = 8
+N_X3_CONFIGS = 9
 
 
 def test_config_table():

@@ -132,7 +132,7 @@ def test_autotune_bucket_table_roundtrip():
     assert 0 <= cfg < len(sa.configs(sa.FORM_F32)) and tf > 0
     assert sa.autotune_lookup(sa.FORM_F32, 500, 400, 300) == (cfg, True)  # same bucket
     cfg_g, _ = sa.autotune_bucket(sa.FORM_BF16X3, 200, 300, 1000, iters=2, grouped=True)
-    assert cfg_g in (0, 1)
+    assert cfg_g in (1, 8)
     with tempfile.TemporaryDirectory() as d:
         path = os.path.join(d, "table.json")
         sa.autotune_save(path)
