@@ -112,6 +112,65 @@ def copy_calibration(steps):
                       "GBps": 2 * 4 * n / (ms * 1e-3) / 1e9}), flush=True)
 
 
+class SclkSampler:
+    """Engine clock during a timed region: a thread reads the current level of
+    the amdgpu sysfs DPM table (pp_dpm_sclk, the line marked '*') every 2 ms.
+    Reports nothing when the file is not readable."""
+
+    def __init__(self):
+        import glob
+        paths = sorted(glob.glob("/sys/class/drm/card*/device/pp_dpm_sclk"))
+        self.path = paths[0] if paths else None
+        self.samples = []
+        self._stop = None
+        self._thread = None
+
+    def _read(self):
+        with open(self.path) as f:
+            for line in f:
+                if line.rstrip().endswith("*"):
+                    return float(line.split(":")[1].strip().rstrip("*").strip().lower()
+                                 .replace("mhz", ""))
+        return None
+
+    def __enter__(self):
+        import threading
+        if self.path is None:
+            return self
+        try:
+            self._read()
+        except (OSError, ValueError, IndexError):
+            self.path = None
+            return self
+        self._stop = threading.Event()
+
+        def run():
+            while not self._stop.wait(0.002):
+                try:
+                    v = self._read()
+                except (OSError, ValueError, IndexError):
+                    v = None
+                if v is not None:
+                    self.samples.append(v)
+
+        self._thread = threading.Thread(target=run, daemon=True)
+        self._thread.start()
+        return self
+
+    def __exit__(self, *exc):
+        if self._thread is not None:
+            self._stop.set()
+            self._thread.join()
+        return False
+
+    def summary(self):
+        if not self.samples:
+            return None
+        v = sorted(self.samples)
+        return {"mean_mhz": sum(v) / len(v), "p50_mhz": v[len(v) // 2], "min_mhz": v[0],
+                "max_mhz": v[-1], "samples": len(v)}
+
+
 def time_form(fn, steps, warmup, stream):
     """Time `steps` back-to-back calls with HIP events on the launch stream."""
     import torch
@@ -359,19 +418,21 @@ def main():
 
     print(f"bench: plain FP32 {n}^3, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
           flush=True)
-    ms = time_form(plain, 1 if args.x3_only else args.steps, 0 if args.x3_only else args.warmup,
-                   stream)
+    with SclkSampler() as clk:
+        ms = time_form(plain, 1 if args.x3_only else args.steps,
+                       0 if args.x3_only else args.warmup, stream)
     tflops = flops / (ms * 1e-3) / 1e12
-    forms = {"f32": {"tflops": tflops, "ms": ms}}
+    forms = {"f32": {"tflops": tflops, "ms": ms, "sclk_sysfs": clk.summary()}}
 
     if not args.no_bf16x3 and sa.lib().sgemm_config_count(sa.FORM_BF16X3) > 0:
         def split():
             sa.sgemm_bf16x3(n, n, n, alpha, A, n, B, n, beta, C, n, stream)
         print("bench: BF16x3", file=sys.stderr, flush=True)
-        ms3 = time_form(split, args.steps, args.warmup, stream)
+        with SclkSampler() as clk3:
+            ms3 = time_form(split, args.steps, args.warmup, stream)
         t3 = flops / (ms3 * 1e-3) / 1e12
         forms["bf16x3"] = {"tflops": t3, "ms": ms3, "speedup_vs_f32": t3 / tflops,
-                           "mfma_tflops_issued": 3 * t3}
+                           "mfma_tflops_issued": 3 * t3, "sclk_sysfs": clk3.summary()}
 
     if not args.no_error:
         # Wide-range inputs (sign * 2^u, u in [-8, 8)), FP64 torch matmul as truth.
