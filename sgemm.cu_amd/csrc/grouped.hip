@@ -6,11 +6,16 @@
 // Schedule.  Every problem is cut into tiles of the config the autotune table
 // picks for its (form, m, n, k) size bucket.  Problems are ordered by the cost
 // of one of their tiles (k * bm * bn, largest first; ties keep submission
-// order), and their tiles are concatenated into one index space.  Workgroup b
-// of the persistent grid processes indices b, b + G, b + 2G, ... so every pass
-// over the grid takes the next G most expensive tiles (longest-processing-time
-// first).  Inside a pass the index is remapped so that each XCD receives a
-// contiguous run of tiles, which mostly share A/B panels in that XCD's L2.
+// order), and their tiles are concatenated into one index space.  The
+// persistent workgroups take tile indices from a device-wide queue (an
+// agent-scope atomic counter in the header), so a workgroup that finishes a
+// cheap tile takes the next one at once: longest-processing-time-first list
+// scheduling, which leaves at most one of the cheapest tiles of imbalance at
+// the end.  (A static stride b, b + G, b + 2G, ... gives workgroup 0 the most
+// expensive tile of every pass; on 512 problems of log-uniform size that
+// imbalance cost 10-20 %.)  The last workgroup to leave resets the counters,
+// so every launch -- and every replay of a captured graph -- starts from zero
+// without a memset.
 //
 // Each tile is computed by the same tile code as the single-problem kernels
 // (tile_f32.hpp / tile_x3.hpp) with the same config, so a problem's result is
@@ -40,7 +45,12 @@ namespace sgemm_amd {
 struct GroupHeader {
     int count;
     int total_tiles;
-    int pad[14];  // keeps the descriptor array 64-byte aligned
+    int pad0[6];
+    // Tile queue, zero between launches: the next tile index to hand out, and
+    // the number of workgroups that have left.  Only atomics touch them.
+    int next_tile;
+    int exited;
+    int pad1[6];  // keeps the descriptor array 64-byte aligned
 };
 static_assert(sizeof(GroupHeader) == 64, "header size");
 
@@ -68,17 +78,28 @@ __device__ inline void run_tile_f32(const Problem& P, int tm, int tn, char* smem
 template <int FORM>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(FORM == kFormF32 ? kGroupedWpeF32 : kGroupedWpeX3)))
-void k_grouped(const GroupHeader* __restrict__ hdr, const Problem* __restrict__ probs) {
+void k_grouped(GroupHeader* __restrict__ hdr, const Problem* __restrict__ probs) {
     __shared__ __attribute__((aligned(16))) char smem[FORM == kFormF32 ? kGroupedLdsF32
                                                                         : kGroupedLdsX3];
+    // The tile index is broadcast through the first word of the tile's own
+    // LDS between tiles: a separate __shared__ word would push the BF16x3
+    // kernel past the 80 KB that two workgroups per CU allow.
+    int* const s_tile = reinterpret_cast<int*>(smem);
     const int count = hdr->count;
     const int total = hdr->total_tiles;
-    const int G = gridDim.x;
-    for (int base = 0; base < total; base += G) {
-        // The last pass may be partial: remap over the live range only.
-        const int live = min(G, total - base);
-        if ((int)blockIdx.x >= live) break;
-        const int w = base + xcd_remap(blockIdx.x, live);
+    const bool leader = threadIdx.x == 0;
+    if (leader)
+        *s_tile = __hip_atomic_fetch_add(&hdr->next_tile, 1, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    int w = *s_tile;
+    while (w < total) {
+        // Take the following index now; its latency hides behind this tile.
+        int next = 0;
+        if (leader)
+            next = __hip_atomic_fetch_add(&hdr->next_tile, 1, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();  // every thread has read the index before the tile reuses the LDS
         int lo = 0, hi = count - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -105,6 +126,20 @@ void k_grouped(const GroupHeader* __restrict__ hdr, const Problem* __restrict__ 
                 default: vec ? X3TileRunner<X3_64x64, true, false>::run(P, tm, tn, smem)
                              : X3TileRunner<X3_64x64, false, false>::run(P, tm, tn, smem); break;
             }
+        }
+        __syncthreads();  // the tile is done with the LDS
+        if (leader) *s_tile = next;
+        __syncthreads();
+        w = *s_tile;
+    }
+    // Every index fetch of this workgroup precedes its exit count, so the
+    // last workgroup out can clear the queue for the next launch.
+    if (leader) {
+        const int gone = __hip_atomic_fetch_add(&hdr->exited, 1, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+        if (gone == (int)gridDim.x - 1) {
+            __hip_atomic_store(&hdr->next_tile, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&hdr->exited, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -263,7 +298,7 @@ int GroupedPlan::set(int count, const sgemm_problem* problems, hipStream_t strea
 }
 
 int GroupedPlan::launch(hipStream_t stream) const {
-    const GroupHeader* hdr = reinterpret_cast<const GroupHeader*>(dev_);
+    GroupHeader* hdr = reinterpret_cast<GroupHeader*>(dev_);
     const Problem* probs =
         reinterpret_cast<const Problem*>(static_cast<const char*>(dev_) + sizeof(GroupHeader));
     (void)hipGetLastError();

@@ -94,6 +94,44 @@ def test_graph_replay_equals_direct_and_tracks_set():
     plan.destroy()
 
 
+@pytest.mark.parametrize("form", [sa.FORM_F32, sa.FORM_BF16X3])
+def test_tile_queue_starts_from_zero_on_every_launch(form):
+    """The persistent workgroups take tiles from a device-side counter that
+    the last workgroup out resets.  Launches with fewer tiles than workgroups,
+    an empty group replayed from a graph, and back-to-back launches with no
+    host synchronisation between them must each see the queue at zero: a
+    launch that started from a stale index would skip tiles."""
+    import torch
+    tol = orc.fp32_tol_mult if form == sa.FORM_F32 else orc.bf16x3_tol_mult
+    stream = torch.cuda.Stream()
+    tiny_cases, tiny_dev, tiny = _make_group([(20, 30, 40)], seed=700)
+    big_cases, big_dev, big = _make_group(_log_uniform_shapes(120, hi=700, seed=8), seed=800)
+    plan = sa.GroupedPlan(128, form)
+    with torch.cuda.stream(stream):
+        plan.set(tiny, stream)
+        plan.run(stream)  # one tile, every other workgroup leaves at once
+        plan.set(big, stream)
+        for _ in range(4):  # back to back on the stream, no host sync between
+            plan.run(stream)
+        stream.synchronize()
+        for c, d in zip(big_cases + tiny_cases, big_dev + tiny_dev):
+            st = check(c, download(d[2]), tol(c.k))
+            assert st["n_fail"] == 0, ((c.m, c.n, c.k), st)
+        plan.capture(stream)
+        plan.set([], stream)
+        plan.replay(stream)  # zero tiles: every workgroup fetches once and leaves
+        for d in big_dev:
+            d[2].fill_(-3.0)
+        plan.set(big, stream)
+        plan.replay(stream)
+        plan.replay(stream)
+        stream.synchronize()
+    for c, d in zip(big_cases, big_dev):
+        st = check(c, download(d[2]), tol(c.k))
+        assert st["n_fail"] == 0, ((c.m, c.n, c.k), st)
+    plan.destroy()
+
+
 def test_one_shot_grouped_and_empty_members():
     shapes = [(100, 90, 80), (0, 5, 5), (7, 0, 3), (64, 64, 0), (129, 33, 257)]
     cases, dev, probs = _make_group(shapes, seed=500, alpha=1.0, beta=0.5)
