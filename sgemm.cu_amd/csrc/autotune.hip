@@ -76,21 +76,24 @@ int heuristic(int form, int grouped, int m, int n, int k) {
             if (m >= 96 && n >= 96) return 5;
             return m >= 96 ? 2 : 1;
         }
-        // Measured on MI355X (square sweep 1024..16384 and the odd shapes):
-        // 128x128x16 with three workgroups per CU (config 5) wins once there
-        // are four or more waves of 128x128 tiles, unless a short last wave
-        // of a few-wave grid idles most CUs; 128x128 with k-major A and two
-        // per CU (config 4) wins from two waves when the last wave is at
-        // least half full; everything smaller goes to 64x64 tiles with
-        // split-K (config 7), which splits k only when the grid has fewer
-        // than two workgroups per CU.  Short k (<= 256) is epilogue-bound:
-        // 128x64 (config 2).
+        // Measured on MI355X (square sweeps 1024..16384, the sizes whose
+        // last round is short, and the odd shapes).  128x128x16 with three
+        // workgroups per CU (768 resident) is the fastest tile from one full
+        // round of 128x128 tiles on.  Its tail-split form (config 9) splits
+        // k for the tiles of a partial last round where the model of
+        // tail_split_count says it pays, and is config 5 elsewhere:
+        // 3584^3 104 -> 136, 4608^3 117 -> 133, 5120^3 123 -> 139,
+        // 9216^3 135 -> 141 TFLOP/s.  Between one round and 1024 tiles, a
+        // last round more than a quarter full goes to 128x128 with k-major
+        // A and two per CU (config 4).  Below one round: config 4 when its
+        // last wave is at least half full, else 64x64 tiles with split-K
+        // (config 7), which splits k only when the grid has fewer than two
+        // workgroups per CU.  Short k (<= 256) is epilogue-bound: 128x64
+        // (config 2).
         const long long t128 = (long long)div_up(m, 128) * div_up(n, 128);
         if (k <= 256 && t128 >= 512) return 2;
-        if (t128 >= 1024) {
-            const long long rem = t128 % 768, waves = div_up(t128, 768LL);
-            return (waves <= 3 && rem > 0 && rem < 128) ? 7 : 5;
-        }
+        if (t128 >= 1024) return 9;
+        if (t128 >= 768) return t128 % 768 <= 192 ? 9 : 4;
         if (t128 >= 512) {
             const long long rem = t128 % 512;
             return (rem == 0 || rem >= 256) ? 4 : 7;
@@ -107,26 +110,42 @@ int heuristic(int form, int grouped, int m, int n, int k) {
     // one of a two-per-CU config gets about 60 % of the CU (measured at
     // 2048^3 with config 8).  The cheapest config wins.  Config 7 splits k
     // when its grid is short of two workgroups per CU (splitk_count).
-    struct Cand { int id, bm, bn, bk, per_cu; double rate; bool split; };
+    // Config 9 is config 2 with a tail split: from one full round on, the
+    // tiles of a partial last round are cut into k-ranges, which costs what
+    // tail_split_count models instead of a whole round; when it would not
+    // split it is config 2 and is left to config 2.
+    enum { kWhole = 0, kSplitK = 1, kTailSplit = 2 };
+    struct Cand { int id, bm, bn, bk, per_cu; double rate; int split; };
     static const Cand cands[] = {
-        {2, 256, 256, 32, 1, 1.00, false}, {5, 256, 128, 16, 2, 0.92, false},
-        {8, 128, 128, 32, 2, 0.84, false}, {3, 256, 128, 32, 1, 0.69, false},
-        {7, 64, 64, 64, 2, 0.47, true},
+        {2, 256, 256, 32, 1, 1.00, kWhole},   {9, 256, 256, 32, 1, 1.00, kTailSplit},
+        {5, 256, 128, 16, 2, 0.92, kWhole},   {8, 128, 128, 32, 2, 0.84, kWhole},
+        {3, 256, 128, 32, 1, 0.69, kWhole},   {7, 64, 64, 64, 2, 0.47, kSplitK},
     };
     const long long cus = device_cu_count();
     int best = 7;
     double best_cost = 0;
     for (const Cand& c : cands) {
         const long long tiles = (long long)div_up(m, c.bm) * div_up(n, c.bn);
-        const long long splits = c.split ? splitk_count(tiles, k, c.bk) : 1;
-        const long long wgs = tiles * splits, slots = cus * c.per_cu;
-        const double work = (double)c.bm * c.bn / splits / c.rate;  // one workgroup, full CU share
-        double cost = (double)(wgs / slots) * c.per_cu * work;
-        const long long rem = wgs % slots;
-        if (rem > 0) {
-            const long long w = div_up(rem, cus);
-            const double u = w >= c.per_cu ? 1.0 : 0.6 + 0.4 * (double)(w - 1) / (c.per_cu - 1);
-            cost += (double)w * work / u;
+        const long long slots = cus * c.per_cu;
+        double cost;
+        if (c.split == kTailSplit) {
+            double tail = 1.0;
+            const int s = tail_split_count(tiles, slots, c.per_cu, k, c.bk, kX3TileSplitBytes,
+                                           2.0 * c.bm * c.bn * k / kX3FlopsPerCuUs, &tail);
+            if (s <= 1) continue;
+            const double round = (double)c.per_cu * c.bm * c.bn / c.rate;
+            cost = (double)(tiles / slots) * round + tail * round;
+        } else {
+            const long long splits = c.split == kSplitK ? splitk_count(tiles, k, c.bk) : 1;
+            const long long wgs = tiles * splits;
+            const double work = (double)c.bm * c.bn / splits / c.rate;  // one workgroup, full CU share
+            cost = (double)(wgs / slots) * c.per_cu * work;
+            const long long rem = wgs % slots;
+            if (rem > 0) {
+                const long long w = div_up(rem, cus);
+                const double u = w >= c.per_cu ? 1.0 : 0.6 + 0.4 * (double)(w - 1) / (c.per_cu - 1);
+                cost += (double)w * work / u;
+            }
         }
         if (best_cost == 0 || cost < best_cost) {
             best = c.id;

@@ -49,6 +49,10 @@ struct SplitK {
     int* counters;    // [tiles], zero between launches
     int splits;
     int kchunk;       // multiple of the tile's BK
+    // Tail split (FP32 kernel only): workgroups [0, dp_tiles) compute whole
+    // tiles [0, dp_tiles) unsplit; the workgroups after them split the
+    // remaining tiles' k ranges.  0: every workgroup is a split unit.
+    int dp_tiles;
 };
 
 // Cache policy of the partials' loads and stores: sc1 (agent coherence).

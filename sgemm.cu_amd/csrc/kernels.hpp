@@ -12,7 +12,8 @@ struct ConfigInfo {
     int bm, bn, bk;  // workgroup tile
     int wm, wn;      // waves along m and n
     const char* name;
-    int splitk;      // 1: the k range is split across workgroups (splitk_count)
+    int splitk;      // 1: the k range is split across workgroups (splitk_count);
+                     // 2: only the tiles of a partial last round are split
 };
 
 // Compute units of the current device (queried once per process).
@@ -21,6 +22,28 @@ int device_cu_count();
 // Number of k-splits for a tiles-sized grid: enough workgroups for two per
 // CU, each split keeping at least four k-tiles, at most 16 splits.
 int splitk_count(long long tiles, int k, int bk);
+
+// Tail split: the r = tiles % slots tiles left after the full rounds of
+// `slots` resident workgroups (per_cu per CU) are cut into s k-ranges.  The
+// s chosen minimises the modelled time of what is left, in rounds of whole
+// tiles: ceil(r s / slots) / s for the split units, plus the partials'
+// traffic ((s + 1) partials of split_bytes through one workgroup at
+// ~32 GB/s) over round_us, the time of one round.  Unsplit, the busiest CU
+// holds ceil(r / CUs) of its per_cu workgroups, which run no faster each
+// (the cost is that over per_cu).  At least four k-tiles per range, at most
+// 16 ranges and 512 MiB of partials; 1 (and *cost = the unsplit cost) when
+// no split beats the unsplit round by 3 %.
+int tail_split_count(long long tiles, long long slots, int per_cu, int k, int bk,
+                     size_t split_bytes, double round_us, double* cost);
+
+// Useful BF16x3 rate of one CU running config 2's 256x256 tile (392 TFLOP/s
+// over 256 CUs at 8192^3), for the length of a round in the tail-split model.
+constexpr double kX3FlopsPerCuUs = 1.53e6;
+// Partials of one 256x256 BF16x3 split: 128 FP32 accumulators x 512 threads.
+constexpr size_t kX3TileSplitBytes = 256 * 256 * 4;
+// Plain FP32 rate of one CU running config 5's tile three at a time (141
+// TFLOP/s over 256 CUs at 8192^3).
+constexpr double kF32FlopsPerCuUs = 0.55e6;
 
 // Split plan for a tiles-sized grid of BK-deep tiles whose split partials
 // take split_bytes per tile: sets sk->splits (1 = no split) and kchunk, and

@@ -7,21 +7,33 @@ namespace sgemm_amd {
 constexpr int kX3GroupN = 8;
 
 // WPE: waves per SIMD the register allocation must allow (T::WPE).
-// SPLIT: the grid is tiles x splits, split index outermost (as sgemm_f32.hip).
+// SPLIT: workgroups [0, sk.dp_tiles) compute tiles [0, sk.dp_tiles) whole;
+// the rest are (tile, split) units of the remaining tiles, split index
+// outermost (as sgemm_f32.hip).
 template <class T, bool VEC, bool SPLIT>
 __global__ __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(T::WPE)))
 void k_sgemm_bf16x3(Problem P, SplitK sk) {
     __shared__ __attribute__((aligned(16))) char smem[T::LDS_BYTES];
-    const int id = xcd_remap(blockIdx.x, gridDim.x);
-    const int tiles = P.tiles_m * P.tiles_n;
-    const int split = SPLIT ? id / tiles : 0;
     int tm, tn;
-    tile_coords(SPLIT ? id - split * tiles : id, P.tiles_m, P.tiles_n, kX3GroupN, tm, tn);
-    X3TileRunner<T, VEC>::run(P, tm, tn, smem, SPLIT ? &sk : nullptr, split);
+    const int b = blockIdx.x;
+    if (!SPLIT || b < sk.dp_tiles) {
+        tile_coords(xcd_remap(b, SPLIT ? sk.dp_tiles : (int)gridDim.x), P.tiles_m, P.tiles_n,
+                    kX3GroupN, tm, tn);
+        X3TileRunner<T, VEC>::run(P, tm, tn, smem);
+    } else {
+        const int units = (int)gridDim.x - sk.dp_tiles;
+        const int u = xcd_remap(b - sk.dp_tiles, units);
+        const int count = units / sk.splits;  // split tiles in this launch
+        const int split = u / count;
+        const int slot = u - split * count;
+        tile_coords(sk.dp_tiles + slot, P.tiles_m, P.tiles_n, kX3GroupN, tm, tn);
+        X3TileRunner<T, VEC>::run(P, tm, tn, smem, &sk, split, slot);
+    }
 }
 
 // Ids 6-7 run the tiles of ids 0-1 with split-K (splitk = 1); id 8 is a
-// 128x128 tile with one accumulator set, two workgroups per CU.
+// 128x128 tile with one accumulator set, two workgroups per CU; id 9 runs
+// the tile of id 2 with a tail split (splitk = 2, as FP32 config 9).
 static const ConfigInfo kBf16x3Configs[] = {
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m16_dual"},
     {64, 64, 64, 2, 2, "bf16x3_64x64x64_w2x2_m16_dual"},
@@ -32,6 +44,7 @@ static const ConfigInfo kBf16x3Configs[] = {
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m16_dual_splitk", 1},
     {64, 64, 64, 2, 2, "bf16x3_64x64x64_w2x2_m16_dual_splitk", 1},
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m32_single_2cu"},
+    {256, 256, 32, 2, 4, "bf16x3_256x256x32_w2x4_m32_single_tailsplit", 2},
 };
 
 int bf16x3_config_count() { return (int)(sizeof(kBf16x3Configs) / sizeof(kBf16x3Configs[0])); }
@@ -42,10 +55,11 @@ static hipError_t launch_x3(const Problem& p, hipStream_t s) {
     Problem q = p;
     q.tiles_m = div_up(p.m, T::BM);
     q.tiles_n = div_up(p.n, T::BN);
+    q.tile_begin = 0;
     const long long tiles = (long long)q.tiles_m * q.tiles_n;
     if (tiles <= 0) return hipSuccess;
     if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
-    SplitK sk{nullptr, nullptr, 1, p.k};
+    SplitK sk{nullptr, nullptr, 1, p.k, 0};
     if (ALLOW_SPLIT) {
         constexpr size_t acc_bytes = T::MF == 16 ? 16 : 64;
         hipError_t e = splitk_plan(s, tiles, p.k, T::BK, T::MB * T::NB * acc_bytes * T::NT, &sk);
@@ -65,11 +79,81 @@ static hipError_t launch_x3(const Problem& p, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Workgroups of the tail-split kernel resident on the device at once.
+template <class T>
+static long long x3_resident_workgroups() {
+    static const long long n = [] {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sgemm_bf16x3<T, true, true>,
+                                                         T::NT, 0) != hipSuccess ||
+            per_cu < 1) {
+            (void)hipGetLastError();
+            per_cu = 1;
+        }
+        return (long long)per_cu * device_cu_count();
+    }();
+    return n;
+}
+
+// Tail split: whole tiles for every full round of resident workgroups, and
+// the r tiles of a partial last round cut into k-ranges (tail_split_count).
+// With one workgroup per CU the tiles of a partial round run no faster than
+// in a full one, so without the split a round with a few tiles costs as much
+// as a full one.
+template <class T>
+static hipError_t launch_x3_tail(const Problem& p, hipStream_t s) {
+    Problem q = p;
+    q.tiles_m = div_up(p.m, T::BM);
+    q.tiles_n = div_up(p.n, T::BN);
+    q.tile_begin = 0;
+    const long long tiles = (long long)q.tiles_m * q.tiles_n;
+    if (tiles <= 0) return hipSuccess;
+    if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
+    const long long slots = x3_resident_workgroups<T>();
+    const long long rem = tiles % slots;
+    constexpr size_t acc_bytes = T::MF == 16 ? 16 : 64;
+    const size_t split_bytes = (size_t)T::MB * T::NB * acc_bytes * T::NT;
+    double tail_cost = 1.0;
+    const int per_cu = (int)(slots / device_cu_count());
+    const int splits =
+        tail_split_count(tiles, slots, per_cu, p.k, T::BK, split_bytes,
+                         per_cu * 2.0 * T::BM * T::BN * p.k / kX3FlopsPerCuUs, &tail_cost);
+    SplitK sk{nullptr, nullptr, 1, p.k, 0};
+    if (splits > 1) {
+        sk.kchunk = div_up(div_up(p.k, splits), T::BK) * T::BK;
+        sk.splits = div_up(p.k, sk.kchunk);
+        if (sk.splits > 1) {
+            const hipError_t e = splitk_workspace(s, (size_t)rem * sk.splits * split_bytes,
+                                                  (int)rem, &sk.partials, &sk.counters);
+            if (e == hipErrorStreamCaptureUnsupported) {
+                sk.splits = 1;
+                sk.kchunk = p.k;
+            } else if (e != hipSuccess) {
+                return e;
+            }
+        }
+    }
+    (void)hipGetLastError();
+    const bool vec = vec4_ok(p);
+    if (sk.splits > 1) {
+        sk.dp_tiles = (int)(tiles - rem);
+        const dim3 grid((unsigned)(tiles - rem + rem * sk.splits));
+        if (vec) hipLaunchKernelGGL((k_sgemm_bf16x3<T, true, true>), grid, dim3(T::NT), 0, s, q, sk);
+        else hipLaunchKernelGGL((k_sgemm_bf16x3<T, false, true>), grid, dim3(T::NT), 0, s, q, sk);
+    } else {
+        const dim3 grid((unsigned)tiles);
+        if (vec) hipLaunchKernelGGL((k_sgemm_bf16x3<T, true, false>), grid, dim3(T::NT), 0, s, q, sk);
+        else hipLaunchKernelGGL((k_sgemm_bf16x3<T, false, false>), grid, dim3(T::NT), 0, s, q, sk);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_bf16x3(int id, const Problem& p, hipStream_t s) {
     if (id < 0 || id >= bf16x3_config_count()) return hipErrorInvalidValue;
     switch (id) {
         case 6: return launch_x3<X3_128x128, true>(p, s);
         case 7: return launch_x3<X3_64x64, true>(p, s);
+        case 9: return launch_x3_tail<X3_256x256_w8>(p, s);
         default: break;
     }
     hipError_t e = hipErrorInvalidValue;

@@ -7,21 +7,37 @@ namespace sgemm_amd {
 // Column bands of this many tile-columns in the launch order.
 constexpr int kGroupN = 8;
 
-// SPLIT: the grid is tiles x splits; workgroup id -> (split, tile) with the
-// split index outermost, so workgroups running together share k-chunks.
+// SPLIT: workgroups [0, sk.dp_tiles) compute tiles [0, sk.dp_tiles) whole;
+// the rest are (tile, split) units of the remaining tiles, with the split
+// index outermost so that workgroups running together share k-chunks.  The
+// dispatcher hands out workgroups in id order, so the split units start as
+// the whole tiles finish: a partial last round of whole tiles becomes a full
+// round of shorter units.
 template <class T, bool VEC, bool SPLIT>
 __global__ __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(T::WPE)))
 void k_sgemm_f32(Problem P, SplitK sk) {
     __shared__ __attribute__((aligned(16))) float smem[T::LDS_FLOATS];
-    const int id = xcd_remap(blockIdx.x, gridDim.x);
-    const int tiles = P.tiles_m * P.tiles_n;
-    const int split = SPLIT ? id / tiles : 0;
     int tm, tn;
-    tile_coords(SPLIT ? id - split * tiles : id, P.tiles_m, P.tiles_n, kGroupN, tm, tn);
-    F32TileRunner<T, VEC>::run(P, tm, tn, smem, SPLIT ? &sk : nullptr, split);
+    const int b = blockIdx.x;
+    if (!SPLIT || b < sk.dp_tiles) {
+        tile_coords(xcd_remap(b, SPLIT ? sk.dp_tiles : (int)gridDim.x), P.tiles_m, P.tiles_n,
+                    kGroupN, tm, tn);
+        F32TileRunner<T, VEC>::run(P, tm, tn, smem);
+    } else {
+        const int units = (int)gridDim.x - sk.dp_tiles;
+        const int u = xcd_remap(b - sk.dp_tiles, units);
+        const int count = units / sk.splits;  // split tiles in this launch
+        const int split = u / count;
+        const int slot = u - split * count;
+        tile_coords(sk.dp_tiles + slot, P.tiles_m, P.tiles_n, kGroupN, tm, tn);
+        F32TileRunner<T, VEC>::run(P, tm, tn, smem, &sk, split, slot);
+    }
 }
 
-// Ids 6-8 run the tiles of ids 0-2 with split-K (splitk = 1).
+// Ids 6-8 run the tiles of ids 0-2 with split-K (splitk = 1).  Id 9 runs
+// the tile of id 5 with a tail split (splitk = 2): whole tiles for every full
+// round of the resident workgroups, split-K for the tiles of the last,
+// partial round.
 static const ConfigInfo kF32Configs[] = {
     {128, 128, 32, 2, 2, "f32_128x128x32_w2x2"},
     {64, 64, 32, 2, 2, "f32_64x64x32_w2x2"},
@@ -32,6 +48,7 @@ static const ConfigInfo kF32Configs[] = {
     {128, 128, 32, 2, 2, "f32_128x128x32_w2x2_splitk", 1},
     {64, 64, 32, 2, 2, "f32_64x64x32_w2x2_splitk", 1},
     {128, 64, 32, 2, 2, "f32_128x64x32_w2x2_splitk", 1},
+    {128, 128, 16, 2, 2, "f32_128x128x16_w2x2_3wpe_tailsplit", 2},
 };
 
 int f32_config_count() { return (int)(sizeof(kF32Configs) / sizeof(kF32Configs[0])); }
@@ -42,10 +59,11 @@ static hipError_t launch_tile(const Problem& p, hipStream_t s) {
     Problem q = p;
     q.tiles_m = div_up(p.m, T::BM);
     q.tiles_n = div_up(p.n, T::BN);
+    q.tile_begin = 0;
     const long long tiles = (long long)q.tiles_m * q.tiles_n;
     if (tiles <= 0) return hipSuccess;
     if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
-    SplitK sk{nullptr, nullptr, 1, p.k};
+    SplitK sk{nullptr, nullptr, 1, p.k, 0};
     if (ALLOW_SPLIT) {
         hipError_t e = splitk_plan(s, tiles, p.k, T::BK, (size_t)T::MB * T::NB * T::NT * 16, &sk);
         if (e != hipSuccess) return e;
@@ -64,12 +82,81 @@ static hipError_t launch_tile(const Problem& p, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Workgroups of kernel K resident on the device at once.
+template <class T>
+static long long resident_workgroups() {
+    static const long long n = [] {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sgemm_f32<T, true, true>,
+                                                         T::NT, 0) != hipSuccess ||
+            per_cu < 1) {
+            (void)hipGetLastError();
+            per_cu = 1;
+        }
+        return (long long)per_cu * device_cu_count();
+    }();
+    return n;
+}
+
+// Tail split: the tiles of full rounds run whole; the r tiles of a partial
+// last round are each cut into the number of k-ranges tail_split_count
+// picks.  Falls back to one launch of whole tiles when there is no partial
+// round, when the split would not pay, or when the stream is being captured
+// and its workspace would have to grow.
+template <class T>
+static hipError_t launch_tail_split(const Problem& p, hipStream_t s) {
+    Problem q = p;
+    q.tiles_m = div_up(p.m, T::BM);
+    q.tiles_n = div_up(p.n, T::BN);
+    q.tile_begin = 0;
+    const long long tiles = (long long)q.tiles_m * q.tiles_n;
+    if (tiles <= 0) return hipSuccess;
+    if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
+    const long long slots = resident_workgroups<T>();
+    const long long rem = tiles % slots;
+    const size_t split_bytes = (size_t)T::MB * T::NB * T::NT * 16;
+    const int per_cu = (int)(slots / device_cu_count());
+    double tail_cost = 1.0;
+    const int splits =
+        tail_split_count(tiles, slots, per_cu, p.k, T::BK, split_bytes,
+                         per_cu * 2.0 * T::BM * T::BN * p.k / kF32FlopsPerCuUs, &tail_cost);
+    SplitK sk{nullptr, nullptr, 1, p.k, 0};
+    if (splits > 1) {
+        sk.kchunk = div_up(div_up(p.k, splits), T::BK) * T::BK;
+        sk.splits = div_up(p.k, sk.kchunk);
+        if (sk.splits > 1) {
+            const hipError_t e = splitk_workspace(s, (size_t)rem * sk.splits * split_bytes,
+                                                  (int)rem, &sk.partials, &sk.counters);
+            if (e == hipErrorStreamCaptureUnsupported) {
+                sk.splits = 1;
+                sk.kchunk = p.k;
+            } else if (e != hipSuccess) {
+                return e;
+            }
+        }
+    }
+    (void)hipGetLastError();  // the status read below must belong to this launch
+    const bool vec = vec4_ok(p);
+    if (sk.splits > 1) {
+        sk.dp_tiles = (int)(tiles - rem);
+        const dim3 grid((unsigned)(tiles - rem + rem * sk.splits));
+        if (vec) hipLaunchKernelGGL((k_sgemm_f32<T, true, true>), grid, dim3(T::NT), 0, s, q, sk);
+        else hipLaunchKernelGGL((k_sgemm_f32<T, false, true>), grid, dim3(T::NT), 0, s, q, sk);
+    } else {
+        const dim3 grid((unsigned)tiles);
+        if (vec) hipLaunchKernelGGL((k_sgemm_f32<T, true, false>), grid, dim3(T::NT), 0, s, q, sk);
+        else hipLaunchKernelGGL((k_sgemm_f32<T, false, false>), grid, dim3(T::NT), 0, s, q, sk);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_f32(int id, const Problem& p, hipStream_t s) {
     if (id < 0 || id >= f32_config_count()) return hipErrorInvalidValue;
     switch (id) {
         case 6: return launch_tile<F32_128x128, true>(p, s);
         case 7: return launch_tile<F32_64x64, true>(p, s);
         case 8: return launch_tile<F32_128x64, true>(p, s);
+        case 9: return launch_tail_split<F32_128x128_k16>(p, s);
         default: break;
     }
     hipError_t e = hipErrorInvalidValue;

@@ -121,9 +121,11 @@ struct F32TileRunner {
 
     // Computes tile (tm, tn) of problem P.  `smem` holds T::LDS_FLOATS floats.
     // All threads of the workgroup must call this together.  With `sk`, only
-    // split `split` of the k range runs here (see SplitK).
+    // split `split` of the k range runs here (see SplitK), and `slot` (unique
+    // per split tile of the launch, below the workspace's tile count) picks
+    // the tile's partials and arrival counter.
     __device__ static void run(const Problem& P, int tm, int tn, float* smem,
-                               const SplitK* sk = nullptr, int split = 0) {
+                               const SplitK* sk = nullptr, int split = 0, int slot = 0) {
         const int t = threadIdx.x;
         const int lane = t & 63, wave = t >> 6;
         const int wm = wave % T::WM, wn = wave / T::WM;
@@ -309,9 +311,8 @@ struct F32TileRunner {
 
         if (sk) {
             // Dump the accumulators in lane order (coalesced 16-byte stores),
-            // then let the last workgroup of this tile reduce in split order;
-            // its own partial comes from its registers.
-            const int tile = tm * P.tiles_n + tn;
+            // then let the last workgroup of this tile reduce in split order.
+            const int tile = slot;
             constexpr int NACC = T::MB * T::NB;
             constexpr int SPLIT_BYTES = NACC * T::NT * 16;
             const __amdgpu_buffer_rsrc_t part =
@@ -324,33 +325,23 @@ struct F32TileRunner {
                     store_agent(part, split * SPLIT_BYTES + ((i * T::NB + j) * T::NT + t) * 16,
                                 acc[i][j]);
             if (!splitk_arrive(*sk, tile, reinterpret_cast<int*>(smem))) return;
-            // One split's loads are all issued before any is waited on.
-            f32x4 sum[T::MB][T::NB];
+            // Sum the partials in split order, this workgroup's own re-read
+            // like the others (bit-identical to its registers), so the sum
+            // reuses the accumulators' registers.  One split's loads are all
+            // issued before any is waited on.
 #pragma unroll
             for (int i = 0; i < T::MB; ++i)
 #pragma unroll
                 for (int j = 0; j < T::NB; ++j)
-                    sum[i][j] = split == 0 ? acc[i][j]
-                                           : load_agent(part, ((i * T::NB + j) * T::NT + t) * 16);
+                    acc[i][j] = load_agent(part, ((i * T::NB + j) * T::NT + t) * 16);
             for (int q = 1; q < sk->splits; ++q) {
-                if (q == split) {
 #pragma unroll
-                    for (int i = 0; i < T::MB; ++i)
+                for (int i = 0; i < T::MB; ++i)
 #pragma unroll
-                        for (int j = 0; j < T::NB; ++j) sum[i][j] += acc[i][j];
-                } else {
-#pragma unroll
-                    for (int i = 0; i < T::MB; ++i)
-#pragma unroll
-                        for (int j = 0; j < T::NB; ++j)
-                            sum[i][j] += load_agent(
-                                part, q * SPLIT_BYTES + ((i * T::NB + j) * T::NT + t) * 16);
-                }
+                    for (int j = 0; j < T::NB; ++j)
+                        acc[i][j] += load_agent(
+                            part, q * SPLIT_BYTES + ((i * T::NB + j) * T::NT + t) * 16);
             }
-#pragma unroll
-            for (int i = 0; i < T::MB; ++i)
-#pragma unroll
-                for (int j = 0; j < T::NB; ++j) acc[i][j] = sum[i][j];
         }
 
         // ---- epilogue: lane holds rows 4g..4g+3 of column l16 of each block.

@@ -150,9 +150,11 @@ struct X3TileRunner {
 
     // All threads of the workgroup must call this together.  With `sk`, only
     // k in [split * kchunk, (split + 1) * kchunk) is accumulated and the
-    // partials are reduced across splits (common.hpp).
+    // partials are reduced across splits (common.hpp); `slot` (unique per
+    // split tile of the launch, below the workspace's tile count) picks the
+    // tile's partials and arrival counter.
     __device__ static void run(const Problem& P, int tm, int tn, char* smem,
-                               const SplitK* sk = nullptr, int split = 0) {
+                               const SplitK* sk = nullptr, int split = 0, int slot = 0) {
         const int t = threadIdx.x;
         const int lane = t & 63, wave = t >> 6;
         const int wm = wave % T::WM, wn = wave / T::WM;
@@ -355,7 +357,7 @@ struct X3TileRunner {
         if (sk) {
             // Fold the cross-term accumulators in, dump this split's sums in
             // lane order, and let the last workgroup of the tile add all
-            // splits in split order (its own from registers).
+            // splits in split order.
             if constexpr (T::DUAL) {
 #pragma unroll
                 for (int i = 0; i < T::MB; ++i)
@@ -368,7 +370,7 @@ struct X3TileRunner {
             constexpr int NV = sizeof(Acc) / 16;  // float4 pieces per accumulator
             constexpr int NACC = T::MB * T::NB * NV;
             constexpr int SPLIT_BYTES = NACC * T::NT * 16;
-            const int tile = tm * P.tiles_n + tn;
+            const int tile = slot;
             const __amdgpu_buffer_rsrc_t part =
                 make_rsrc(sk->partials + (size_t)tile * sk->splits * NACC * T::NT * 4,
                           (long long)sk->splits * SPLIT_BYTES);
@@ -385,40 +387,31 @@ struct X3TileRunner {
                     for (int v = 0; v < NV; ++v)
                         store_agent(part, split * SPLIT_BYTES + off(i, j, v), piece(i, j, v));
             if (!splitk_arrive(*sk, tile, reinterpret_cast<int*>(smem))) return;
-            Acc sum[T::MB][T::NB];
+            // This workgroup's own partial is re-read like the others
+            // (bit-identical to its registers), so the sum reuses the
+            // accumulators' registers.
 #pragma unroll
             for (int i = 0; i < T::MB; ++i)
 #pragma unroll
                 for (int j = 0; j < T::NB; ++j)
 #pragma unroll
                     for (int v = 0; v < NV; ++v) {
-                        const f32x4 x = split == 0 ? piece(i, j, v) : load_agent(part, off(i, j, v));
+                        const f32x4 x = load_agent(part, off(i, j, v));
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) sum[i][j][4 * v + r] = x[r];
+                        for (int r = 0; r < 4; ++r) acc[i][j][4 * v + r] = x[r];
                     }
             for (int q = 1; q < sk->splits; ++q) {
-                if (q == split) {
 #pragma unroll
-                    for (int i = 0; i < T::MB; ++i)
+                for (int i = 0; i < T::MB; ++i)
 #pragma unroll
-                        for (int j = 0; j < T::NB; ++j) sum[i][j] += acc[i][j];
-                } else {
+                    for (int j = 0; j < T::NB; ++j)
 #pragma unroll
-                    for (int i = 0; i < T::MB; ++i)
+                        for (int v = 0; v < NV; ++v) {
+                            const f32x4 x = load_agent(part, q * SPLIT_BYTES + off(i, j, v));
 #pragma unroll
-                        for (int j = 0; j < T::NB; ++j)
-#pragma unroll
-                            for (int v = 0; v < NV; ++v) {
-                                const f32x4 x = load_agent(part, q * SPLIT_BYTES + off(i, j, v));
-#pragma unroll
-                                for (int r = 0; r < 4; ++r) sum[i][j][4 * v + r] += x[r];
-                            }
-                }
+                            for (int r = 0; r < 4; ++r) acc[i][j][4 * v + r] += x[r];
+                        }
             }
-#pragma unroll
-            for (int i = 0; i < T::MB; ++i)
-#pragma unroll
-                for (int j = 0; j < T::NB; ++j) acc[i][j] = sum[i][j];
         }
 
         // Epilogue.  16x16 block: lane holds rows 4*(lane/16) + 0..3 of column

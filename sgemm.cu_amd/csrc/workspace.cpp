@@ -3,7 +3,8 @@
 // Split-K partials and the per-tile arrival counters live in device memory
 // owned by the library: one workspace per stream (launches on one stream are
 // ordered, so they can share it; launches on different streams get their own).
-// The counters are zeroed once when allocated and every split-K launch leaves
+// The counters are zeroed once when allocated (on the owning stream, ahead of
+// its first launch) and every split-K launch leaves
 // them zero again, so no memset is needed per launch (and none lands in a
 // captured graph).  A stream that is being captured never allocates: a
 // split-K launch whose workspace would have to grow runs unsplit instead, so
@@ -59,6 +60,27 @@ int splitk_count(long long tiles, int k, int bk) {
     return (int)std::max(1LL, s);
 }
 
+int tail_split_count(long long tiles, long long slots, int per_cu, int k, int bk,
+                     size_t split_bytes, double round_us, double* cost) {
+    const long long cus = std::max(1LL, slots / std::max(1, per_cu));
+    const long long rem = slots > 0 ? tiles % slots : 0;
+    *cost = (double)((rem + cus - 1) / cus) / std::max(1, per_cu);
+    if (tiles < slots || rem == 0 || round_us <= 0) return 1;
+    const int max_s = std::min(16, k / (4 * bk));
+    int best = 1;
+    for (int s = 2; s <= max_s; ++s) {
+        if ((double)rem * s * (double)split_bytes > 512.0 * 1024 * 1024) break;
+        const long long rounds = (rem * s + slots - 1) / slots;
+        const double traffic_us = (double)(s + 1) * (double)split_bytes / 32.0e3;
+        const double c = (double)rounds / s + traffic_us / round_us;
+        if (c < 0.97 * *cost) {
+            *cost = c;
+            best = s;
+        }
+    }
+    return best;
+}
+
 hipError_t splitk_plan(hipStream_t stream, long long tiles, int k, int bk, size_t split_bytes,
                        SplitK* sk) {
     sk->partials = nullptr;
@@ -108,7 +130,12 @@ hipError_t splitk_workspace(hipStream_t stream, size_t partial_bytes, int tiles,
             (void)hipGetLastError();
             return hipErrorOutOfMemory;
         }
-        if (hipMemset(c, 0, (size_t)n * sizeof(int)) != hipSuccess) {
+        // Zeroed on the stream that will use the counters: a plain hipMemset
+        // runs on the null stream, which a non-blocking stream does not wait
+        // for, so a launch right after it could read recycled memory as
+        // arrival counts (no workgroup would see itself last, and the tiles
+        // would never be written).  Never reached under capture.
+        if (hipMemsetAsync(c, 0, (size_t)n * sizeof(int), stream) != hipSuccess) {
             (void)hipFree(c);
             return hipErrorUnknown;
         }

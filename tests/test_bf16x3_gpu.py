@@ -16,12 +16,13 @@ import pytest
 
 import oracle as orc
 import sgemm_amd as sa
-from gpu_util import Case, assert_ok, check, compact, download, log, run_form, upload
+from gpu_util import (Case, assert_ok, check, compact, download, log, padding_untouched, run_form,
+                      upload)
 
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gemm_cases.json")
-N_X3_CONFIGS = 9
+N_X3_CONFIGS = 10
 
 
 def test_config_table():
@@ -159,3 +160,92 @@ def test_splitk_integer_kat_bit_exact():
     b = run_form(c, sa.FORM_BF16X3, 6)
     assert np.array_equal(a, b)
     assert_ok(check(c, b, orc.bf16x3_tol_mult(k)), "split-K integer KAT")
+
+
+# ------------------------------------------------------------ tail split
+X3_TAIL_CONFIG = 9  # config 2's 256x256 tile; the tiles of a partial last round split k
+
+
+@pytest.mark.parametrize("m,n,k,ld_pad", [
+    (4100, 3900, 1000, 3),   # 17x16 = 272 tiles of 256x256: 16 past one 256-CU round
+    (7168, 7168, 2048, 0),   # 784 tiles: 16 past three rounds
+])
+def test_tail_split_deterministic_and_in_tolerance(m, n, k, ld_pad):
+    """In the BF16x3 tolerance against FP64 (every entry below 2^24 of them,
+    sampled above), bit-identical on repeat with another problem of the
+    same shape in between (the partial slots are reused)."""
+    c = Case(m, n, k, lda=m + ld_pad, ldb=k + ld_pad, ldc=m + ld_pad, seed=m + n + k,
+             alpha=1.25, beta=0.5)
+    first = run_form(c, sa.FORM_BF16X3, X3_TAIL_CONFIG)
+    if m * n <= (1 << 24):
+        assert_ok(check(c, first, orc.bf16x3_tol_mult(k)), f"tail split {(m, n, k)}")
+    else:
+        _sampled_check_x3(c, first, n_samples=4096, seed=k)
+    assert padding_untouched(first, c.C0, m, n, c.ldc)
+    other = Case(m, n, k, lda=m + ld_pad, ldb=k + ld_pad, ldc=m + ld_pad, kind="wide",
+                 seed=m + n + k + 7, alpha=1.0, beta=0.0)
+    out = run_form(other, sa.FORM_BF16X3, X3_TAIL_CONFIG)
+    _sampled_check_x3(other, out, n_samples=2048, seed=k + 1)
+    again = run_form(c, sa.FORM_BF16X3, X3_TAIL_CONFIG)
+    assert np.array_equal(first, again), (m, n, k)
+
+
+def _sampled_check_x3(c, out, n_samples, seed):
+    rng = np.random.default_rng(seed)
+    ii = np.concatenate([rng.integers(0, c.m, n_samples), [0, c.m - 1, 0, c.m - 1]]).astype(np.int32)
+    jj = np.concatenate([rng.integers(0, c.n, n_samples), [0, 0, c.n - 1, c.n - 1]]).astype(np.int32)
+    ref, scale = orc.dgemm_entries(c.k, c.alpha, c.A, c.lda, c.B, c.ldb, c.beta, c.C0, c.ldc, ii, jj)
+    got = out[ii + jj.astype(np.int64) * c.ldc].astype(np.float64)
+    err = np.abs(got - ref)
+    tol = orc.bf16x3_tol_mult(c.k) * scale
+    assert np.all(np.isfinite(got)) and np.all(err <= tol), float(np.max(err / tol))
+
+
+def test_tail_split_integer_kat_equals_unsplit():
+    """Integer inputs are BF16-exact and every partial sum is an exact
+    integer, so the tail split equals config 2 (no split) bit for bit."""
+    m, n, k = 4352, 4096, 2048  # 272 tiles: 16 split tiles
+    c = Case(m, n, k, kind="int", seed=19, alpha=1.5, beta=0.5)
+    a = run_form(c, sa.FORM_BF16X3, 2)
+    b = run_form(c, sa.FORM_BF16X3, X3_TAIL_CONFIG)
+    assert np.array_equal(a, b)
+
+
+def test_tail_split_graph_replays_and_capture_fallback():
+    """Captured after a warm-up call on the stream, the tail split replays
+    bit-identically to its eager result; captured on a fresh stream (no
+    workspace yet) it runs config 2's kernel unsplit, bit for bit."""
+    import torch
+    m, n, k = 4352, 4096, 1024
+    c = Case(m, n, k, seed=78, alpha=1.0, beta=0.0)
+    dA, dB, dC = c.device()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        sa.run_config(1, X3_TAIL_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, s)
+        s.synchronize()
+        eager = download(dC).copy()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            sa.run_config(1, X3_TAIL_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m,
+                          torch.cuda.current_stream())
+    for rep in range(3):
+        dC.fill_(5.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(download(dC), eager), rep
+    sa.run_config(1, 2, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m)
+    torch.cuda.synchronize()
+    unsplit = download(dC).copy()
+    assert not np.array_equal(unsplit, eager)  # the tail tiles were split
+    fresh = torch.cuda.Stream()
+    g2 = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g2, stream=fresh):
+        sa.run_config(1, X3_TAIL_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m,
+                      torch.cuda.current_stream())
+    for rep in range(2):
+        dC.fill_(-2.0)
+        torch.cuda.synchronize()
+        g2.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(download(dC), unsplit), rep

@@ -18,12 +18,13 @@ import pytest
 
 import oracle as orc
 import sgemm_amd as sa
-from gpu_util import Case, assert_ok, check, compact, download, log, run_form, upload
+from gpu_util import (Case, assert_ok, check, compact, download, log, padding_untouched, run_form,
+                      upload)
 
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gemm_cases.json")
-N_F32_CONFIGS = 9
+N_F32_CONFIGS = 10
 
 
 @pytest.fixture(scope="module")
@@ -299,6 +300,38 @@ def test_splitk_graph_replays_equal_eager():
         assert np.array_equal(download(dC), eager), rep
 
 
+def test_splitk_first_call_on_busy_nonblocking_stream():
+    """A stream's split-K workspace is allocated by its first split call and
+    its arrival counters are zeroed on that stream, ahead of the launch (a
+    null-stream memset is not ordered before work on torch's non-blocking
+    streams).  Recycled device memory (a freed grouped plan's non-zero
+    descriptors) and a device kept busy by another stream must not leak into
+    the counts: a stale count would leave tiles of the result unwritten.
+    torch hands out pooled streams, so a trial's stream is new to the library
+    only while the pool has unused ones; the serving tests, whose servers
+    create their own streams, cover the same path."""
+    import torch
+    m, n, k = 1350, 143, 1915  # 64x64 split-K tiles (config 7), six splits
+    busy_c = Case(4096, 4096, 4096, seed=3, alpha=1.0, beta=0.0)
+    bA, bB, bC = busy_c.device()
+    for trial in range(3):
+        junk = [sa.problem(300, 200, 100, 1.0, bA, 300, bB, 100, 0.0, bC, 300)] * 64
+        plan = sa.GroupedPlan(256, sa.FORM_F32)
+        plan.set(junk)
+        plan.run()
+        torch.cuda.synchronize()
+        plan.destroy()
+        c = Case(m, n, k, seed=40 + trial, alpha=1.25, beta=0.5)
+        dA, dB, dC = c.device()
+        busy, fresh = torch.cuda.Stream(), torch.cuda.Stream()
+        torch.cuda.synchronize()
+        sa.sgemm_ex(4096, 4096, 4096, 1.0, bA, 4096, bB, 4096, 0.0, bC, 4096, busy)
+        log(f"split-K first call on a fresh stream, trial {trial}")
+        sa.run_config(0, 7, m, n, k, c.alpha, dA, m, dB, k, c.beta, dC, m, fresh)
+        torch.cuda.synchronize()
+        assert_ok(check(c, download(dC), orc.fp32_tol_mult(k)), f"fresh stream trial {trial}")
+
+
 def test_splitk_capture_without_warmup_runs_unsplit():
     """A split-K launch captured on a stream that has no workspace yet does
     not allocate inside the capture: it runs the unsplit tile (config 0's
@@ -318,6 +351,90 @@ def test_splitk_capture_without_warmup_runs_unsplit():
         dC.fill_(-1.0)
         torch.cuda.synchronize()
         g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(download(dC), unsplit), rep
+
+
+# ------------------------------------------------------------ tail split
+TAIL_CONFIG = 9  # config 5's tile; the tiles of a partial last round split k
+
+
+@pytest.mark.parametrize("m,n,k,ld_pad", [
+    (3584, 3584, 1024, 0),   # 784 tiles of 128x128: 16 left over a 768-slot round
+    (3000, 4100, 777, 5),    # ragged edges and k chunks, padded leading dimensions
+    (8192, 8064, 1000, 0),   # 4032 tiles: 192 left over
+])
+def test_tail_split_deterministic_and_in_tolerance(m, n, k, ld_pad):
+    """Whole tiles for the full rounds, split-K for the last partial round:
+    in FP32 tolerance against FP64 (every entry below 2^26 of them, sampled
+    above), bit-identical on repeat with another problem of the same shape
+    in between (the split partial slots are reused), and every entry within
+    twice the tolerance of config 5 (same tile, no split)."""
+    import torch
+    c = Case(m, n, k, lda=m + ld_pad, ldb=k + ld_pad, ldc=m + ld_pad, seed=m + n + k,
+             alpha=1.25, beta=0.5)
+    first = run_form(c, sa.FORM_F32, TAIL_CONFIG)
+    if m * n <= (1 << 24):
+        assert_ok(check(c, first, orc.fp32_tol_mult(k)), f"tail split {(m, n, k)}")
+    else:
+        _sampled_check(c, first, n_samples=4096, seed=k)
+    other = Case(m, n, k, lda=m + ld_pad, ldb=k + ld_pad, ldc=m + ld_pad, seed=m + n + k + 7,
+                 alpha=1.0, beta=0.0)
+    out = run_form(other, sa.FORM_F32, TAIL_CONFIG)
+    _sampled_check(other, out, n_samples=2048, seed=k + 1)
+    again = run_form(c, sa.FORM_F32, TAIL_CONFIG)
+    assert np.array_equal(first, again), (m, n, k)
+    plain = run_form(c, sa.FORM_F32, 5)
+    got, ref = compact(first, m, n, c.ldc), compact(plain, m, n, c.ldc)
+    # |got - plain| <= |got - truth| + |truth - plain| <= 2 tol * scale, with
+    # scale = |alpha| |A| |B| + |beta| |C0| from the naive kernel on the GPU
+    dA, dB = upload(np.abs(c.A)), upload(np.abs(c.B))
+    scale = upload(np.abs(c.C0) * abs(c.beta))
+    assert sa.sgemm_basic(m, n, k, abs(c.alpha), dA, c.lda, dB, c.ldb, 1.0, scale, c.ldc) == 0
+    torch.cuda.synchronize()
+    bound = 2 * orc.fp32_tol_mult(k) * 1.001 * compact(download(scale), m, n, c.ldc)
+    assert np.all(np.abs(got.astype(np.float64) - ref) <= bound)
+    assert padding_untouched(first, c.C0, m, n, c.ldc)
+
+
+def test_tail_split_graph_replays_and_capture_fallback():
+    """Captured after a warm-up call on the stream, the tail split replays
+    bit-identically to its eager result.  Captured on a fresh stream (no
+    workspace yet), it runs config 5's unsplit kernel, so every replay
+    equals config 5 bit for bit."""
+    import torch
+    m = n = 3584
+    k = 2048
+    c = Case(m, n, k, seed=77, alpha=1.0, beta=0.0)
+    dA, dB, dC = c.device()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        sa.run_config(0, TAIL_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, s)
+        s.synchronize()
+        eager = download(dC).copy()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            sa.run_config(0, TAIL_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m,
+                          torch.cuda.current_stream())
+    for rep in range(3):
+        dC.fill_(5.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(download(dC), eager), rep
+    sa.run_config(0, 5, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m)
+    torch.cuda.synchronize()
+    unsplit = download(dC).copy()
+    assert not np.array_equal(unsplit, eager)  # the tail tiles were split
+    fresh = torch.cuda.Stream()
+    g2 = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g2, stream=fresh):
+        sa.run_config(0, TAIL_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m,
+                      torch.cuda.current_stream())
+    for rep in range(2):
+        dC.fill_(-2.0)
+        torch.cuda.synchronize()
+        g2.replay()
         torch.cuda.synchronize()
         assert np.array_equal(download(dC), unsplit), rep
 
