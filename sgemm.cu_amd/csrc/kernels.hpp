@@ -19,9 +19,12 @@ struct ConfigInfo {
 // Compute units of the current device (queried once per process).
 int device_cu_count();
 
-// Number of k-splits for a tiles-sized grid: enough workgroups for two per
-// CU, each split keeping at least four k-tiles, at most 16 splits.
-int splitk_count(long long tiles, int k, int bk);
+// Number of k-splits for a tiles-sized grid.  BF16x3 (f32 = false): enough
+// workgroups for two per CU, each split keeping at least four k-tiles.  Plain
+// FP32: about 4.5 workgroups per CU (the nearest whole split count), each
+// split keeping at least eight k-tiles -- its small tiles hide latency only
+// with several workgroups per CU.  At most 16 splits either way.
+int splitk_count(long long tiles, int k, int bk, bool f32 = false);
 
 // Tail split: the r = tiles % slots tiles left after the full rounds of
 // `slots` resident workgroups (per_cu per CU) are cut into s k-ranges.  The
@@ -49,7 +52,7 @@ constexpr double kF32FlopsPerCuUs = 0.55e6;
 // take split_bytes per tile: sets sk->splits (1 = no split) and kchunk, and
 // for a split grid the workspace pointers.
 hipError_t splitk_plan(hipStream_t stream, long long tiles, int k, int bk, size_t split_bytes,
-                       SplitK* sk);
+                       SplitK* sk, bool f32 = false);
 
 // Per-stream split-K workspace (partials + zeroed per-tile counters), grown on
 // demand; the first call on a stream, or a larger problem, allocates.

@@ -65,7 +65,8 @@ static hipError_t launch_tile(const Problem& p, hipStream_t s) {
     if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
     SplitK sk{nullptr, nullptr, 1, p.k, 0};
     if (ALLOW_SPLIT) {
-        hipError_t e = splitk_plan(s, tiles, p.k, T::BK, (size_t)T::MB * T::NB * T::NT * 16, &sk);
+        hipError_t e =
+            splitk_plan(s, tiles, p.k, T::BK, (size_t)T::MB * T::NB * T::NT * 16, &sk, true);
         if (e != hipSuccess) return e;
     }
     const bool split = ALLOW_SPLIT && sk.splits > 1;
