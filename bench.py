@@ -187,6 +187,24 @@ def time_form(fn, steps, warmup, stream):
     return start.elapsed_time(stop) / steps  # ms per call
 
 
+def per_call(fn, calls, stream):
+    """Each call timed on its own (one HIP event pair per call, no cache flush):
+    the median, and the mean of the second half as the reference's harness
+    reports it (benchmark.cu:141-183)."""
+    import numpy as np
+    import torch
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(calls)]
+    for a, b in evs:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    t = [a.elapsed_time(b) for a, b in evs]
+    return {"calls": calls, "p50_ms": float(np.median(t)),
+            "mean_second_half_ms": float(np.mean(t[calls // 2:]))}
+
+
 def error_vs_fp64(C, ref64, scale64):
     """Componentwise error |C - ref| / (|alpha||A||B| + |beta||C0|) and the
     plain relative error over ref != 0 (max and mean of both)."""
@@ -423,6 +441,8 @@ def main():
                        0 if args.x3_only else args.warmup, stream)
     tflops = flops / (ms * 1e-3) / 1e12
     forms = {"f32": {"tflops": tflops, "ms": ms, "sclk_sysfs": clk.summary()}}
+    if not args.x3_only:
+        forms["f32"]["per_call"] = per_call(plain, min(args.steps, 20), stream)
 
     if not args.no_bf16x3 and sa.lib().sgemm_config_count(sa.FORM_BF16X3) > 0:
         def split():
@@ -432,7 +452,8 @@ def main():
             ms3 = time_form(split, args.steps, args.warmup, stream)
         t3 = flops / (ms3 * 1e-3) / 1e12
         forms["bf16x3"] = {"tflops": t3, "ms": ms3, "speedup_vs_f32": t3 / tflops,
-                           "mfma_tflops_issued": 3 * t3, "sclk_sysfs": clk3.summary()}
+                           "mfma_tflops_issued": 3 * t3, "sclk_sysfs": clk3.summary(),
+                           "per_call": per_call(split, min(args.steps, 20), stream)}
 
     if not args.no_error:
         # Wide-range inputs (sign * 2^u, u in [-8, 8)), FP64 torch matmul as truth.
