@@ -87,7 +87,7 @@ int heuristic(int form, int grouped, int m, int n, int k) {
         // last round more than a quarter full goes to 128x128 with k-major
         // A and two per CU (config 4).  Below one round: config 4 when its
         // last wave is at least half full, else 64x64 tiles with split-K
-        // (config 7), which splits k until the grid holds about 4.5
+        // (config 7), which splits k until the grid holds about four
         // workgroups per CU (splitk_count).  Short k (<= 256) is
         // epilogue-bound: 128x64 (config 2).
         const long long t128 = (long long)div_up(m, 128) * div_up(n, 128);

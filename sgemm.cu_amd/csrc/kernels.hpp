@@ -19,11 +19,11 @@ struct ConfigInfo {
 // Compute units of the current device (queried once per process).
 int device_cu_count();
 
-// Number of k-splits for a tiles-sized grid.  BF16x3 (f32 = false): enough
-// workgroups for two per CU, each split keeping at least four k-tiles.  Plain
-// FP32: about 4.5 workgroups per CU (the nearest whole split count), each
-// split keeping at least eight k-tiles -- its small tiles hide latency only
-// with several workgroups per CU.  At most 16 splits either way.
+// Number of k-splits for a tiles-sized grid, each split keeping at least
+// four k-tiles, at most 16.  BF16x3 (f32 = false): split only a grid of fewer
+// than two workgroups per CU, up to two per CU.  Plain FP32: the split count
+// nearest to four workgroups per CU -- its small tiles hide latency only with
+// several workgroups per CU.
 int splitk_count(long long tiles, int k, int bk, bool f32 = false);
 
 // Tail split: the r = tiles % slots tiles left after the full rounds of

@@ -53,11 +53,13 @@ int device_cu_count() {
 
 int splitk_count(long long tiles, int k, int bk, bool f32) {
     if (f32) {
-        // Measured on the square sizes 1024..3072 (config 7): 1152^3 63 -> 81,
-        // 1280^3 78 -> 91, 1536^3 84 -> 98 TFLOP/s over the two-per-CU rule.
-        const double want = 4.5 * device_cu_count() / (double)std::max(tiles, 1LL);
+        // Measured on the square sizes 1024..3072 (config 7): 1152^3 63 -> 91,
+        // 1280^3 78 -> 90, 1536^3 84 -> 97 TFLOP/s over the two-per-CU rule,
+        // and no loss on 512 log-uniform problems of 32..2048 launched one by
+        // one (a cap of eight k-tiles per split cost those 17 %).
+        const double want = 4.0 * device_cu_count() / (double)std::max(tiles, 1LL);
         const long long s = std::min({(long long)(want + 0.5),
-                                      (long long)std::max(1, k / (8 * bk)), 16LL});
+                                      (long long)std::max(1, k / (4 * bk)), 16LL});
         return (int)std::max(1LL, s);
     }
     const long long target = 2LL * device_cu_count();
