@@ -108,8 +108,9 @@ int heuristic(int form, int grouped, int m, int n, int k) {
     // config's relative speed measured at 8192^3.  In a last, partial round
     // a CU holds w < per_cu workgroups, which then run faster each: a lone
     // one of a two-per-CU config gets about 60 % of the CU (measured at
-    // 2048^3 with config 8).  The cheapest config wins.  Config 7 splits k
-    // when its grid is short of two workgroups per CU (splitk_count).
+    // 2048^3 with config 8).  The cheapest config wins.  Configs 7 and 10
+    // split k when their grid is short of two workgroups per CU
+    // (splitk_count); 10 only into splits of at least 16 k-tiles.
     // Config 9 is config 2 with a tail split: from one full round on, the
     // tiles of a partial last round are cut into k-ranges, which costs what
     // tail_split_count models instead of a whole round; when it would not
@@ -120,6 +121,7 @@ int heuristic(int form, int grouped, int m, int n, int k) {
         {2, 256, 256, 32, 1, 1.00, kWhole},   {9, 256, 256, 32, 1, 1.00, kTailSplit},
         {5, 256, 128, 16, 2, 0.92, kWhole},   {8, 128, 128, 32, 2, 0.84, kWhole},
         {3, 256, 128, 32, 1, 0.69, kWhole},   {7, 64, 64, 64, 2, 0.47, kSplitK},
+        {10, 128, 128, 32, 2, 0.70, kSplitK},  // config 8's rate less its split's partials
     };
     const long long cus = device_cu_count();
     int best = 7;
@@ -136,7 +138,10 @@ int heuristic(int form, int grouped, int m, int n, int k) {
             const double round = (double)c.per_cu * c.bm * c.bn / c.rate;
             cost = (double)(tiles / slots) * round + tail * round;
         } else {
-            const long long splits = c.split == kSplitK ? splitk_count(tiles, k, c.bk) : 1;
+            const long long splits =
+                c.split == kSplitK
+                    ? splitk_count(tiles, k, c.bk, false, c.id == 10 ? kX3WideSplitKtiles : 4)
+                    : 1;
             const long long wgs = tiles * splits;
             const double work = (double)c.bm * c.bn / splits / c.rate;  // one workgroup, full CU share
             cost = (double)(wgs / slots) * c.per_cu * work;

@@ -20,11 +20,11 @@ struct ConfigInfo {
 int device_cu_count();
 
 // Number of k-splits for a tiles-sized grid, each split keeping at least
-// four k-tiles, at most 16.  BF16x3 (f32 = false): split only a grid of fewer
-// than two workgroups per CU, up to two per CU.  Plain FP32: the split count
-// nearest to four workgroups per CU -- its small tiles hide latency only with
-// several workgroups per CU.
-int splitk_count(long long tiles, int k, int bk, bool f32 = false);
+// min_ktiles k-tiles, at most 16.  BF16x3 (f32 = false): split only a grid of
+// fewer than two workgroups per CU, up to two per CU.  Plain FP32: the split
+// count nearest to four workgroups per CU -- its small tiles hide latency
+// only with several workgroups per CU.
+int splitk_count(long long tiles, int k, int bk, bool f32 = false, int min_ktiles = 4);
 
 // Tail split: the r = tiles % slots tiles left after the full rounds of
 // `slots` resident workgroups (per_cu per CU) are cut into s k-ranges.  The
@@ -44,6 +44,10 @@ int tail_split_count(long long tiles, long long slots, int per_cu, int k, int bk
 constexpr double kX3FlopsPerCuUs = 1.53e6;
 // Partials of one 256x256 BF16x3 split: 128 FP32 accumulators x 512 threads.
 constexpr size_t kX3TileSplitBytes = 256 * 256 * 4;
+// Minimum k-tiles per split of BF16x3 config 10 (the 128x128 tile of config
+// 8 with split-K): 1536^3 116 -> 179, 2048^3 218 -> 241 TFLOP/s, while four
+// k-tiles per split ran 1024^3 at half the 64x64 split config's speed.
+constexpr int kX3WideSplitKtiles = 16;
 // Plain FP32 rate of one CU running config 5's tile three at a time (141
 // TFLOP/s over 256 CUs at 8192^3).
 constexpr double kF32FlopsPerCuUs = 0.55e6;
@@ -52,7 +56,7 @@ constexpr double kF32FlopsPerCuUs = 0.55e6;
 // take split_bytes per tile: sets sk->splits (1 = no split) and kchunk, and
 // for a split grid the workspace pointers.
 hipError_t splitk_plan(hipStream_t stream, long long tiles, int k, int bk, size_t split_bytes,
-                       SplitK* sk, bool f32 = false);
+                       SplitK* sk, bool f32 = false, int min_ktiles = 4);
 
 // Per-stream split-K workspace (partials + zeroed per-tile counters), grown on
 // demand; the first call on a stream, or a larger problem, allocates.

@@ -33,7 +33,9 @@ void k_sgemm_bf16x3(Problem P, SplitK sk) {
 
 // Ids 6-7 run the tiles of ids 0-1 with split-K (splitk = 1); id 8 is a
 // 128x128 tile with one accumulator set, two workgroups per CU; id 9 runs
-// the tile of id 2 with a tail split (splitk = 2, as FP32 config 9).
+// the tile of id 2 with a tail split (splitk = 2, as FP32 config 9); id 10
+// runs the tile of id 8 with split-K, each split at least 16 k-tiles deep
+// (shorter splits of this tile lose to the 64x64 one).
 static const ConfigInfo kBf16x3Configs[] = {
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m16_dual"},
     {64, 64, 64, 2, 2, "bf16x3_64x64x64_w2x2_m16_dual"},
@@ -45,12 +47,13 @@ static const ConfigInfo kBf16x3Configs[] = {
     {64, 64, 64, 2, 2, "bf16x3_64x64x64_w2x2_m16_dual_splitk", 1},
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m32_single_2cu"},
     {256, 256, 32, 2, 4, "bf16x3_256x256x32_w2x4_m32_single_tailsplit", 2},
+    {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m32_single_2cu_splitk", 1},
 };
 
 int bf16x3_config_count() { return (int)(sizeof(kBf16x3Configs) / sizeof(kBf16x3Configs[0])); }
 const ConfigInfo& bf16x3_config(int id) { return kBf16x3Configs[id]; }
 
-template <class T, bool ALLOW_SPLIT>
+template <class T, bool ALLOW_SPLIT, int MIN_KTILES = 4>
 static hipError_t launch_x3(const Problem& p, hipStream_t s) {
     Problem q = p;
     q.tiles_m = div_up(p.m, T::BM);
@@ -62,7 +65,8 @@ static hipError_t launch_x3(const Problem& p, hipStream_t s) {
     SplitK sk{nullptr, nullptr, 1, p.k, 0};
     if (ALLOW_SPLIT) {
         constexpr size_t acc_bytes = T::MF == 16 ? 16 : 64;
-        hipError_t e = splitk_plan(s, tiles, p.k, T::BK, T::MB * T::NB * acc_bytes * T::NT, &sk);
+        hipError_t e = splitk_plan(s, tiles, p.k, T::BK, T::MB * T::NB * acc_bytes * T::NT, &sk,
+                                   false, MIN_KTILES);
         if (e != hipSuccess) return e;
     }
     const bool split = ALLOW_SPLIT && sk.splits > 1;
@@ -154,6 +158,7 @@ hipError_t launch_bf16x3(int id, const Problem& p, hipStream_t s) {
         case 6: return launch_x3<X3_128x128, true>(p, s);
         case 7: return launch_x3<X3_64x64, true>(p, s);
         case 9: return launch_x3_tail<X3_256x256_w8>(p, s);
+        case 10: return launch_x3<X3_128x128_2cu, true, kX3WideSplitKtiles>(p, s);
         default: break;
     }
     hipError_t e = hipErrorInvalidValue;

@@ -51,7 +51,7 @@ int device_cu_count() {
     return cus;
 }
 
-int splitk_count(long long tiles, int k, int bk, bool f32) {
+int splitk_count(long long tiles, int k, int bk, bool f32, int min_ktiles) {
     if (f32) {
         // Measured on the square sizes 1024..3072 (config 7): 1152^3 63 -> 91,
         // 1280^3 78 -> 90, 1536^3 84 -> 97 TFLOP/s over the two-per-CU rule,
@@ -59,13 +59,13 @@ int splitk_count(long long tiles, int k, int bk, bool f32) {
         // one (a cap of eight k-tiles per split cost those 17 %).
         const double want = 4.0 * device_cu_count() / (double)std::max(tiles, 1LL);
         const long long s = std::min({(long long)(want + 0.5),
-                                      (long long)std::max(1, k / (4 * bk)), 16LL});
+                                      (long long)std::max(1, k / (min_ktiles * bk)), 16LL});
         return (int)std::max(1LL, s);
     }
     const long long target = 2LL * device_cu_count();
     if (tiles >= target) return 1;
     long long s = target / std::max(tiles, 1LL);
-    const long long max_by_k = std::max(1, k / (4 * bk));
+    const long long max_by_k = std::max(1, k / (min_ktiles * bk));
     s = std::min({s, max_by_k, 16LL});
     return (int)std::max(1LL, s);
 }
@@ -92,10 +92,10 @@ int tail_split_count(long long tiles, long long slots, int per_cu, int k, int bk
 }
 
 hipError_t splitk_plan(hipStream_t stream, long long tiles, int k, int bk, size_t split_bytes,
-                       SplitK* sk, bool f32) {
+                       SplitK* sk, bool f32, int min_ktiles) {
     sk->partials = nullptr;
     sk->counters = nullptr;
-    sk->splits = splitk_count(tiles, k, bk, f32);
+    sk->splits = splitk_count(tiles, k, bk, f32, min_ktiles);
     sk->kchunk = div_up(div_up(k, sk->splits), bk) * bk;
     sk->splits = div_up(k, sk->kchunk);  // every split gets some k
     if (sk->splits <= 1) return hipSuccess;

@@ -22,7 +22,7 @@ from gpu_util import (Case, assert_ok, check, compact, download, log, padding_un
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gemm_cases.json")
-N_X3_CONFIGS = 10
+N_X3_CONFIGS = 11
 
 
 def test_config_table():
@@ -133,7 +133,7 @@ def test_error_within_fp32_bound_at_8192():
 
 
 # ------------------------------------------------------------------- split-K
-@pytest.mark.parametrize("config", (6, 7))
+@pytest.mark.parametrize("config", (6, 7, 10))
 def test_splitk_deterministic_and_in_tolerance(config):
     """Split-K configs (tiles of configs 0 and 1): partials summed in split
     order, so repeated launches are bit-identical; a different problem of the
@@ -160,6 +160,9 @@ def test_splitk_integer_kat_bit_exact():
     b = run_form(c, sa.FORM_BF16X3, 6)
     assert np.array_equal(a, b)
     assert_ok(check(c, b, orc.bf16x3_tol_mult(k)), "split-K integer KAT")
+    # config 10 splits config 8's tile (16 k-tiles per split at least)
+    c = Case(256, 384, 4096, kind="int", seed=10, alpha=1.5, beta=0.5)
+    assert np.array_equal(run_form(c, sa.FORM_BF16X3, 8), run_form(c, sa.FORM_BF16X3, 10))
 
 
 # ------------------------------------------------------------ tail split
