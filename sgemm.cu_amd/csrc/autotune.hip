@@ -94,6 +94,12 @@ int heuristic(int form, int grouped, int m, int n, int k) {
         if (k <= 256 && t128 >= 512) return 2;
         if (t128 >= 1024) return 9;
         if (t128 >= 768) return t128 % 768 <= 192 ? 9 : 4;
+        // Config 5's tile with split-K (config 10) from 300 tiles of 128x128
+        // and between 180 and 255, where the 64x64 tiles leave their last
+        // round short: 1792^3 91 -> 105, 2304^3 100 -> 120, 2560^3 108 ->
+        // 121, 3200^3 119 -> 130 TFLOP/s.  At 256 tiles (2048^3) the 64x64
+        // tiles fill exactly four per CU and win (118 vs 101).
+        if (k >= 1024 && (t128 >= 300 || (t128 >= 180 && t128 < 256))) return 10;
         if (t128 >= 512) {
             const long long rem = t128 % 512;
             return (rem == 0 || rem >= 256) ? 4 : 7;

@@ -37,7 +37,8 @@ void k_sgemm_f32(Problem P, SplitK sk) {
 // Ids 6-8 run the tiles of ids 0-2 with split-K (splitk = 1).  Id 9 runs
 // the tile of id 5 with a tail split (splitk = 2): whole tiles for every full
 // round of the resident workgroups, split-K for the tiles of the last,
-// partial round.
+// partial round.  Id 10 runs the tile of id 5 with split-K over the whole
+// grid, each split at least eight k-tiles deep.
 static const ConfigInfo kF32Configs[] = {
     {128, 128, 32, 2, 2, "f32_128x128x32_w2x2"},
     {64, 64, 32, 2, 2, "f32_64x64x32_w2x2"},
@@ -49,12 +50,13 @@ static const ConfigInfo kF32Configs[] = {
     {64, 64, 32, 2, 2, "f32_64x64x32_w2x2_splitk", 1},
     {128, 64, 32, 2, 2, "f32_128x64x32_w2x2_splitk", 1},
     {128, 128, 16, 2, 2, "f32_128x128x16_w2x2_3wpe_tailsplit", 2},
+    {128, 128, 16, 2, 2, "f32_128x128x16_w2x2_3wpe_splitk", 1},
 };
 
 int f32_config_count() { return (int)(sizeof(kF32Configs) / sizeof(kF32Configs[0])); }
 const ConfigInfo& f32_config(int id) { return kF32Configs[id]; }
 
-template <class T, bool ALLOW_SPLIT>
+template <class T, bool ALLOW_SPLIT, int MIN_KTILES = 4>
 static hipError_t launch_tile(const Problem& p, hipStream_t s) {
     Problem q = p;
     q.tiles_m = div_up(p.m, T::BM);
@@ -66,7 +68,8 @@ static hipError_t launch_tile(const Problem& p, hipStream_t s) {
     SplitK sk{nullptr, nullptr, 1, p.k, 0};
     if (ALLOW_SPLIT) {
         hipError_t e =
-            splitk_plan(s, tiles, p.k, T::BK, (size_t)T::MB * T::NB * T::NT * 16, &sk, true);
+            splitk_plan(s, tiles, p.k, T::BK, (size_t)T::MB * T::NB * T::NT * 16, &sk, true,
+                        MIN_KTILES);
         if (e != hipSuccess) return e;
     }
     const bool split = ALLOW_SPLIT && sk.splits > 1;
@@ -158,6 +161,7 @@ hipError_t launch_f32(int id, const Problem& p, hipStream_t s) {
         case 7: return launch_tile<F32_64x64, true>(p, s);
         case 8: return launch_tile<F32_128x64, true>(p, s);
         case 9: return launch_tail_split<F32_128x128_k16>(p, s);
+        case 10: return launch_tile<F32_128x128_k16, true, 8>(p, s);
         default: break;
     }
     hipError_t e = hipErrorInvalidValue;

@@ -24,7 +24,7 @@ from gpu_util import (Case, assert_ok, check, compact, download, log, padding_un
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gemm_cases.json")
-N_F32_CONFIGS = 10
+N_F32_CONFIGS = 11
 
 
 @pytest.fixture(scope="module")
@@ -255,7 +255,7 @@ def test_sweep_shapes_sampled_fp64_and_full_vs_naive(m, n, k):
 
 
 # ------------------------------------------------------------------- split-K
-SPLIT_CONFIGS = (6, 7, 8)
+SPLIT_CONFIGS = (6, 7, 8, 10)
 
 
 @pytest.mark.parametrize("config", SPLIT_CONFIGS)
