@@ -39,18 +39,16 @@ typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 // registers and so allows larger wave tiles.  MID stages the next k-tile
 // right after the first k-step's MFMAs instead of after the last one; it pays
 // when two independent workgroups share a CU and hurts a lone wave per SIMD,
-// whose loads then have less time to land.  SCHED applies x3_schedule's
-// interleaving hints to the full k-tiles.  WPE_ > 0 caps the registers so
+// whose loads then have less time to land.  WPE_ > 0 caps the registers so
 // that many waves per SIMD fit (two 4-wave workgroups per CU: their barriers
 // fall at different times, so one's MFMAs fill the other's bubbles).
 template <int BM_, int BN_, int BK_, int WM_, int WN_, int MF_ = 16, bool DUAL_ = true,
-          bool MID_ = false, bool SCHED_ = false, int WPE_ = 0>
+          bool MID_ = false, int WPE_ = 0>
 struct X3Tile {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
     static constexpr int MF = MF_;
     static constexpr bool DUAL = DUAL_;
     static constexpr bool MID = MID_;
-    static constexpr bool SCHED = SCHED_;
     // Waves per SIMD the kernel's registers must allow: two independent
     // workgroups per CU for the MID configs or when WPE_ asks for it, else
     // whatever the tile needs.
@@ -104,39 +102,6 @@ __device__ inline void split4(f32x4 x, bf16x4_t& hi, bf16x4_t& lo) {
         hi[2 * p + 1] = h[1];
         lo[2 * p] = l[0];
         lo[2 * p + 1] = l[1];
-    }
-}
-
-// Instruction-group order for the compiler's scheduler in a full k-tile
-// (__builtin_amdgcn_sched_group_barrier: mask, count, sync id).  Masks: 0x008
-// MFMA, 0x002 VALU, 0x020 VMEM read, 0x100 DS read, 0x200 DS write.  The next
-// k-tile's global loads go first; each k-step's operand reads precede its
-// MFMAs; during the last k-step every MFMA is followed by a share of the
-// next k-tile's split (VALU) and one of its LDS stores, so the matrix core
-// keeps receiving work while the VALU and LDS ports are busy.  (Measured on
-// the 256x256 tile: a few percent over the unhinted schedule; on the
-// one-wave-per-SIMD tiles the hints lose, so only config 2 uses them.)
-template <class T>
-__device__ inline void x3_schedule() {
-    constexpr int STEPS = T::BK / T::KSTEP;
-    constexpr int MF_STEP = T::MB * T::NB * 3;                // MFMAs per k-step
-    constexpr int RD_STEP = (T::MB + T::NB) * 2;              // b128 operand reads per k-step
-    constexpr int GLOADS = T::A_SUB * 4 + T::B_VEC;           // 16-byte global loads
-    constexpr int WRITES = T::A_SUB * 4 * 2 + T::B_VEC * 2;   // LDS stores of hi and lo
-    constexpr int VALU = (T::A_SUB * 16 + T::B_VEC * 4) * 3;  // split: ~3 per element
-    constexpr int VPER = (VALU + MF_STEP - 1) / MF_STEP;
-    __builtin_amdgcn_sched_group_barrier(0x020, GLOADS, 0);
-#pragma unroll
-    for (int s = 0; s < STEPS; ++s) {
-        __builtin_amdgcn_sched_group_barrier(0x100, RD_STEP, 0);
-#pragma unroll
-        for (int i = 0; i < MF_STEP; ++i) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            if (s + 1 == STEPS) {
-                __builtin_amdgcn_sched_group_barrier(0x002, VPER, 0);
-                if (i < WRITES) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
-            }
-        }
     }
 }
 
@@ -277,7 +242,7 @@ struct X3TileRunner {
             const bool more = NEXT_FULL || kt + 1 < ktiles;
             const int knext = kbeg + (kt + 1) * BK;
             if (more) gload(knext);
-            if constexpr (NEXT_FULL && !T::SCHED) __builtin_amdgcn_sched_barrier(0);
+            if constexpr (NEXT_FULL) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int kk = 0; kk < BK; kk += T::KSTEP) {
                 bf16x8_t ah[T::MB], al[T::MB], bh[T::NB], bl[T::NB];
@@ -322,19 +287,18 @@ struct X3TileRunner {
                 constexpr int kStoreAt = T::MID ? 0 : BK - T::KSTEP;
                 if (kk == kStoreAt) {
                     if constexpr (NEXT_FULL) {
-                        // Without scheduling hints, keep the staging code a
-                        // region of its own (as a branch would): letting the
-                        // scheduler mix it into the MFMA section raises
-                        // register pressure past the budget.
-                        if constexpr (!T::SCHED) __builtin_amdgcn_sched_barrier(0);
+                        // Keep the staging code a region of its own (as a
+                        // branch would): letting the scheduler mix it into
+                        // the MFMA section raises register pressure past the
+                        // budget.
+                        __builtin_amdgcn_sched_barrier(0);
                         sstore_impl(next_stage, knext, std::false_type{});
-                        if constexpr (!T::SCHED) __builtin_amdgcn_sched_barrier(0);
+                        __builtin_amdgcn_sched_barrier(0);
                     } else if (more) {
                         sstore(next_stage, knext);
                     }
                 }
             }
-            if constexpr (NEXT_FULL && T::SCHED) x3_schedule<T>();
         };
         // k-tiles whose successor is full run the branch-free body; the
         // last one or two run the general body.  (Two loops in sequence, not
@@ -461,11 +425,13 @@ struct X3TileRunner {
 // Config ids of the BF16x3 form (the order is the ABI's config id).
 using X3_128x128 = X3Tile<128, 128, 32, 2, 2>;
 using X3_64x64 = X3Tile<64, 64, 64, 2, 2>;
-using X3_256x256_w8 = X3Tile<256, 256, 32, 2, 4, 32, false, false, true>;
+// 16x16x32 MFMAs on the 256x256 tile: 389 -> 413 TFLOP/s at 8192^3 and
+// 385 -> 398 at 16384^3 over 32x32x16 with scheduling hints.
+using X3_256x256_w8 = X3Tile<256, 256, 32, 2, 4, 16, false>;
 using X3_256x128_w4 = X3Tile<256, 128, 32, 2, 2, 32, false>;
 using X3_128x128_m32 = X3Tile<128, 128, 32, 2, 2, 32, true>;
 using X3_256x128_k16 = X3Tile<256, 128, 16, 2, 2, 32, false, true>;
-using X3_128x128_2cu = X3Tile<128, 128, 32, 2, 2, 32, false, false, false, 2>;
+using X3_128x128_2cu = X3Tile<128, 128, 32, 2, 2, 32, false, false, 2>;
 
 template <class F>
 __host__ __device__ inline bool dispatch_x3(int id, F&& f) {
