@@ -48,8 +48,9 @@ def parse():
                     help="arrival window per swept rate")
     ap.add_argument("--serve-chunk-gflop", type=float, default=0.0,
                     help="server chunk threshold in GFLOP (0 = library default, < 0 = off)")
-    ap.add_argument("--serve-rates", type=str, default="0.25,0.5,0.75,0.9",
-                    help="arrival rates as fractions of the measured closed-loop capacity")
+    ap.add_argument("--serve-rates", type=str, default="0.25,0.5,0.75,0.9,1.0",
+                    help="arrival rates as fractions of the measured closed-loop capacity "
+                         "(at 1.0 the queue does not settle: its p99 grows with the window)")
     ap.add_argument("--plain-only", action="store_true",
                     help="time only the plain FP32 kernel (profiling runs)")
     ap.add_argument("--x3-only", action="store_true",
