@@ -368,15 +368,18 @@ int sgemm_autotune_bucket(int form, int grouped, int m, int n, int k, int iters,
             // warm-up, then the timed repetitions
             int st = once();
             if (st != SGEMM_STATUS_SUCCESS) continue;
-            hipEventRecord(e0, s);
-            for (int r = 0; r < iters; ++r) once();
-            hipEventRecord(e1, s);
-            if (hipEventSynchronize(e1) != hipSuccess) {
+            if (hipEventRecord(e0, s) != hipSuccess) {
                 status = SGEMM_STATUS_HIP_ERROR;
                 break;
             }
+            for (int r = 0; r < iters && st == SGEMM_STATUS_SUCCESS; ++r) st = once();
             float ms = 0.f;
-            hipEventElapsedTime(&ms, e0, e1);
+            if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                status = SGEMM_STATUS_HIP_ERROR;
+                break;
+            }
+            if (st != SGEMM_STATUS_SUCCESS || !(ms > 0.f)) continue;
             const float tf = (float)(flops * copies * iters / (ms * 1e-3) / 1e12);
             if (best < 0 || tf > best_tf) {
                 best = c;
@@ -384,17 +387,18 @@ int sgemm_autotune_bucket(int form, int grouped, int m, int n, int k, int iters,
             }
         }
         if (plan) {
-            hipStreamSynchronize(s);
+            (void)hipStreamSynchronize(s);
             plan->release();
             delete plan;
         }
-        if (Cc) hipFree(Cc);
+        if (Cc) (void)hipFree(Cc);
     }
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    hipFree(A);
-    hipFree(B);
-    hipFree(C);
+    // Teardown: failures here cannot change the result already measured.
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(A);
+    (void)hipFree(B);
+    (void)hipFree(C);
     if (status == SGEMM_STATUS_SUCCESS && best < 0) status = SGEMM_STATUS_HIP_ERROR;
     if (status == SGEMM_STATUS_SUCCESS) {
         Table& t = table();

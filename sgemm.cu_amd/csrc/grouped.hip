@@ -152,8 +152,9 @@ int persistent_grid(int form) {
     static int grid[2] = {0, 0};
     std::call_once(once, [] {
         int dev = 0, cus = 256;
-        if (hipGetDevice(&dev) == hipSuccess)
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = 256;
         int nb = 1;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_grouped<kFormF32>, 256, 0) !=
                 hipSuccess || nb < 1)
@@ -231,11 +232,11 @@ int GroupedPlan::create(GroupedPlan** out, int capacity, int form) {
 }
 
 void GroupedPlan::release() {
-    if (exec_) hipGraphExecDestroy(exec_);
-    if (graph_) hipGraphDestroy(graph_);
-    if (uploaded_) hipEventDestroy(uploaded_);
-    if (dev_) hipFree(dev_);
-    if (staging_) hipHostFree(staging_);
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (uploaded_) (void)hipEventDestroy(uploaded_);
+    if (dev_) (void)hipFree(dev_);
+    if (staging_) (void)hipHostFree(staging_);
     exec_ = nullptr;
     graph_ = nullptr;
     uploaded_ = nullptr;
@@ -316,11 +317,11 @@ int GroupedPlan::run(hipStream_t stream) const {
 
 int GroupedPlan::capture(hipStream_t stream) {
     if (exec_) {
-        hipGraphExecDestroy(exec_);
+        (void)hipGraphExecDestroy(exec_);
         exec_ = nullptr;
     }
     if (graph_) {
-        hipGraphDestroy(graph_);
+        (void)hipGraphDestroy(graph_);
         graph_ = nullptr;
     }
     (void)hipGetLastError();
@@ -330,7 +331,7 @@ int GroupedPlan::capture(hipStream_t stream) {
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(stream, &g);
     if (st != SGEMM_STATUS_SUCCESS || e != hipSuccess || g == nullptr) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return SGEMM_STATUS_HIP_ERROR;
     }
     graph_ = g;
@@ -406,7 +407,7 @@ int sgemm_grouped_plan_info(sgemm_grouped_plan plan, int* count, int* total_tile
 int sgemm_grouped_plan_destroy(sgemm_grouped_plan plan) {
     if (plan == nullptr) return SGEMM_STATUS_INVALID_VALUE;
     GroupedPlan* p = reinterpret_cast<GroupedPlan*>(plan);
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
     p->release();
     delete p;
     return SGEMM_STATUS_SUCCESS;

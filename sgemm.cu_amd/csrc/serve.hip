@@ -181,15 +181,15 @@ class Server {
         cv_.notify_all();
         if (worker_.joinable()) worker_.join();
         for (Slot& s : slots_) {
-            if (s.stream) hipStreamSynchronize(s.stream);
+            if (s.stream) (void)hipStreamSynchronize(s.stream);
             if (s.plan) {
                 s.plan->release();
                 delete s.plan;
             }
-            if (s.done) hipEventDestroy(s.done);
-            if (s.stream) hipStreamDestroy(s.stream);
+            if (s.done) (void)hipEventDestroy(s.done);
+            if (s.stream) (void)hipStreamDestroy(s.stream);
         }
-        if (pool_) hipFree(pool_);
+        if (pool_) (void)hipFree(pool_);
     }
 
     int submit(const sgemm_request& r, uint64_t* id) {
@@ -343,7 +343,7 @@ class Server {
         for (Slot& s : slots_)
             if (s.busy && s.done) {
                 lock.unlock();
-                hipEventSynchronize(s.done);
+                (void)hipEventSynchronize(s.done);  // an error here leaves nothing to wait for
                 lock.lock();
             }
     }
@@ -458,7 +458,9 @@ class Server {
         if (launch(s) != SGEMM_STATUS_SUCCESS) {
             for (Req* q : s.batch) q->status = SGEMM_STATUS_HIP_ERROR;
             s.mock_done_at = 0;  // retire at once
-            if (cfg_.executor == SGEMM_EXECUTOR_GPU) hipEventRecord(s.done, s.stream);
+            // The slot retires when s.done completes; if even this record
+            // fails, the event still holds an earlier (completed) record.
+            if (cfg_.executor == SGEMM_EXECUTOR_GPU) (void)hipEventRecord(s.done, s.stream);
         }
         return true;
     }
@@ -485,7 +487,7 @@ class Server {
         if (launch_chunk(s) != SGEMM_STATUS_SUCCESS) {
             q->status = SGEMM_STATUS_HIP_ERROR;
             s.mock_done_at = 0;
-            if (cfg_.executor == SGEMM_EXECUTOR_GPU) hipEventRecord(s.done, s.stream);
+            if (cfg_.executor == SGEMM_EXECUTOR_GPU) (void)hipEventRecord(s.done, s.stream);  // as above
         }
         return true;
     }
