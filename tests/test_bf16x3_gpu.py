@@ -97,6 +97,17 @@ def test_beta_zero_nan_c_and_k_zero():
     assert np.array_equal(compact(out, 40, 30, c.ldc), np.float32(0.5) * compact(c.C0, 40, 30, c.ldc))
 
 
+@pytest.mark.parametrize("config,m,n,k", [(6, 190, 77, 4096), (7, 190, 77, 4096),
+                                           (10, 190, 77, 4096), (9, 4352, 4096, 1024)])
+def test_beta_zero_nan_c_with_split_k(config, m, n, k):
+    """beta == 0 never reads C on the split paths (split-K and the tail
+    split's reduction run the same epilogue)."""
+    c = Case(m, n, k, seed=7, beta=0.0, c_fill=np.nan)
+    out = run_form(c, sa.FORM_BF16X3, config)
+    assert not np.isnan(compact(out, m, n, c.ldc)).any()
+    _sampled_check_x3(c, out, n_samples=2048, seed=config)
+
+
 @pytest.mark.parametrize("n", [1024, 2048])
 def test_wide_range_error_report(n):
     """Config-3 style inputs (sign * 2^u, u in [-8, 8)): both forms against FP64.

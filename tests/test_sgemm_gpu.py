@@ -134,6 +134,19 @@ def test_beta_zero_does_not_read_nan_c():
         assert_ok(st, f"beta=0 NaN C config {config}")
 
 
+@pytest.mark.parametrize("config,m,n,k", [(6, 190, 77, 2000), (7, 190, 77, 2000),
+                                           (8, 190, 77, 2000), (10, 190, 77, 2000),
+                                           (9, 3584, 3584, 1024)])
+def test_beta_zero_nan_c_with_split_k(config, m, n, k):
+    """beta == 0 never reads C on the split paths either: the last workgroup
+    of a split tile runs the same epilogue (a NaN read would reach every
+    entry of the tile)."""
+    c = Case(m, n, k, seed=6, alpha=1.25, beta=0.0, c_fill=np.nan)
+    out = run_form(c, sa.FORM_F32, config)
+    assert not np.isnan(compact(out, m, n, c.ldc)).any()
+    _sampled_check(c, out, n_samples=2048, seed=config)
+
+
 def test_k_zero_and_alpha_zero_scale_c():
     for (k, alpha, beta) in [(0, 1.5, 0.5), (33, 0.0, 0.5), (0, 1.0, 0.0)]:
         c = Case(70, 20, k, seed=5, alpha=alpha, beta=beta)
