@@ -45,9 +45,9 @@ static const ConfigInfo kBf16x3Configs[] = {
     {256, 128, 16, 2, 2, "bf16x3_256x128x16_w2x2_m32_single_mid"},
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m16_dual_splitk", 1},
     {64, 64, 64, 2, 2, "bf16x3_64x64x64_w2x2_m16_dual_splitk", 1},
-    {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m32_single_2cu"},
+    {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m16_single_2cu"},
     {256, 256, 32, 2, 4, "bf16x3_256x256x32_w2x4_m16_single_tailsplit", 2},
-    {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m32_single_2cu_splitk", 1},
+    {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m16_single_2cu_splitk", 1},
 };
 
 int bf16x3_config_count() { return (int)(sizeof(kBf16x3Configs) / sizeof(kBf16x3Configs[0])); }
