@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-error", action="store_true", help="skip the FP64 error report")
     ap.add_argument("--cpu-cols", type=int, default=0,
                     help="output columns of the CPU baseline sample (0 = all: the whole problem)")
+    ap.add_argument("--large-size", type=int, default=16384,
+                    help="also time both forms, and their FP64 error, at this size (0 = skip)")
     ap.add_argument("--no-grouped", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--group-count", type=int, default=512)
@@ -216,6 +218,58 @@ def error_vs_fp64(C, ref64, scale64):
     rel = d[nz] / ref64[nz].abs()
     return {"max_comp": float(comp.max()), "mean_comp": float(comp.mean()),
             "max_rel": float(rel.max()), "mean_rel": float(rel.mean())}
+
+
+def wide_error(sa, torch, n, with_x3, stream):
+    """Wide-range inputs (sign * 2^u, u in [-8, 8)), FP64 torch matmul as
+    truth: the componentwise and relative error of each form at n^3."""
+    dev = "cuda"
+    g = torch.Generator(device=dev)
+    g.manual_seed(1)
+
+    def wide():
+        u = torch.rand(n * n, device=dev, generator=g, dtype=torch.float64) * 16 - 8
+        s = torch.where(torch.rand(n * n, device=dev, generator=g) < 0.5, -1.0, 1.0)
+        return (s.double() * torch.exp2(u)).float()
+    Aw, Bw = wide(), wide()
+    Am = Aw.view(n, n).t().double()   # column-major -> logical (m, k)
+    Bm = Bw.view(n, n).t().double()
+    ref = (Am @ Bm).t().contiguous().view(-1)
+    scale = (Am.abs() @ Bm.abs()).t().contiguous().view(-1)
+    del Am, Bm
+    Cw = torch.zeros(n * n, device=dev)
+    out = {}
+    sa.sgemm_ex(n, n, n, 1.0, Aw, n, Bw, n, 0.0, Cw, n, stream)
+    out["f32"] = error_vs_fp64(Cw, ref, scale)
+    if with_x3:
+        sa.sgemm_bf16x3(n, n, n, 1.0, Aw, n, Bw, n, 0.0, Cw, n, stream)
+        out["bf16x3"] = error_vs_fp64(Cw, ref, scale)
+    del ref, scale, Aw, Bw, Cw
+    torch.cuda.empty_cache()
+    return out
+
+
+def forms_at(sa, torch, n, steps, warmup, stream, with_error):
+    """Both forms at n^3 (uniform [-1, 1) inputs, alpha=1.5, beta=0.5), and
+    their wide-range error against FP64."""
+    A = torch.rand(n * n, device="cuda") * 2 - 1
+    B = torch.rand(n * n, device="cuda") * 2 - 1
+    C = torch.rand(n * n, device="cuda") * 2 - 1
+    flops = 2.0 * n ** 3
+    ms = time_form(lambda: sa.sgemm_ex(n, n, n, 1.5, A, n, B, n, 0.5, C, n, stream),
+                   steps, warmup, stream)
+    ms3 = time_form(lambda: sa.sgemm_bf16x3(n, n, n, 1.5, A, n, B, n, 0.5, C, n, stream),
+                    steps, warmup, stream)
+    del A, B, C
+    torch.cuda.empty_cache()
+    out = {"size": n, "steps": steps,
+           "f32": {"ms": ms, "tflops": flops / (ms * 1e-3) / 1e12},
+           "bf16x3": {"ms": ms3, "tflops": flops / (ms3 * 1e-3) / 1e12},
+           "speedup_bf16x3_vs_f32": ms / ms3}
+    if with_error:
+        for name, e in wide_error(sa, torch, n, True, stream).items():
+            out[name]["error_wide"] = e
+    return out
 
 
 def cpu_baseline(size: int, cols: int):
@@ -457,28 +511,16 @@ def main():
                            "per_call": per_call(split, min(args.steps, 20), stream)}
 
     if not args.no_error:
-        # Wide-range inputs (sign * 2^u, u in [-8, 8)), FP64 torch matmul as truth.
         print("bench: error vs FP64", file=sys.stderr, flush=True)
-        g = torch.Generator(device=dev)
-        g.manual_seed(1)
-        def wide():
-            u = torch.rand(n * n, device=dev, generator=g, dtype=torch.float64) * 16 - 8
-            s = torch.where(torch.rand(n * n, device=dev, generator=g) < 0.5, -1.0, 1.0)
-            return (s.double() * torch.exp2(u)).float()
-        Aw, Bw = wide(), wide()
-        Am = Aw.view(n, n).t().double()   # column-major -> logical (m, k)
-        Bm = Bw.view(n, n).t().double()
-        ref = (Am @ Bm).t().contiguous().view(-1)
-        scale = (Am.abs() @ Bm.abs()).t().contiguous().view(-1)
-        del Am, Bm
-        Cw = torch.zeros_like(C)
-        sa.sgemm_ex(n, n, n, 1.0, Aw, n, Bw, n, 0.0, Cw, n, stream)
-        forms["f32"]["error_wide"] = error_vs_fp64(Cw, ref, scale)
-        if "bf16x3" in forms:
-            sa.sgemm_bf16x3(n, n, n, 1.0, Aw, n, Bw, n, 0.0, Cw, n, stream)
-            forms["bf16x3"]["error_wide"] = error_vs_fp64(Cw, ref, scale)
-        del ref, scale, Aw, Bw, Cw
-        torch.cuda.empty_cache()
+        errs = wide_error(sa, torch, n, "bf16x3" in forms, stream)
+        for name, e in errs.items():
+            forms[name]["error_wide"] = e
+
+    large = None
+    if args.large_size > 0 and not args.no_bf16x3 and not args.plain_only and not args.x3_only:
+        print(f"bench: both forms at {args.large_size}^3", file=sys.stderr, flush=True)
+        large = forms_at(sa, torch, args.large_size, min(args.steps, 5), min(args.warmup, 2),
+                         stream, not args.no_error)
 
     grouped = None
     if not args.no_grouped:
@@ -520,6 +562,7 @@ def main():
         "config": {"workload": f"sgemm column-major M=N=K={n}, lda=ldb=ldc={n}, alpha=1.5, "
                                f"beta=0.5", "m": n, "n": n, "k": n},
         "forms": forms,
+        "forms_large": large,
         "grouped": grouped,
         "serving": serving,
         "roofline": roofline,
