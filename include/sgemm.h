@@ -93,13 +93,13 @@ const char* sgemm_hip_error_string(void);
 
 /* As sgemm(), on `stream`, returning a status.  The config is chosen by the
  * autotune table for (m, n, k, SGEMM_FORM_F32).  Split-K configs (chosen for
- * grids with fewer than two workgroups per CU, and the tail-split config for
- * grids whose last round of resident workgroups is partial) use a device
+ * grids that would hold few workgroups per CU) and the tail-split config
+ * (for grids whose last round of resident workgroups is partial) use a device
  * workspace owned by the library, one per stream, allocated on the first such
- * call and kept
- * until the process ends; a call on a stream under graph capture never
- * allocates and runs unsplit if the workspace is too small.  Results of a
- * split-K config are deterministic (partials are summed in split order). */
+ * call and kept until the process ends; a call on a stream under graph
+ * capture never allocates and runs unsplit if the workspace is too small.
+ * Results of a split config are deterministic (partials are summed in split
+ * order). */
 int sgemm_ex(int m, int n, int k, const float* alpha, const float* A, int lda, const float* B,
              int ldb, const float* beta, float* C, int ldc, sgemm_stream_t stream);
 
