@@ -152,7 +152,11 @@ int sgemm_grouped_plan_set(sgemm_grouped_plan plan, int count, const sgemm_probl
 int sgemm_grouped_plan_set_configs(sgemm_grouped_plan plan, int count,
                                    const sgemm_problem* problems, const int* config_ids,
                                    sgemm_stream_t stream);
-/* One persistent kernel launch computing every problem of the plan. */
+/* One persistent kernel launch computing every problem of the plan.  Its
+ * workgroups take tiles from a queue in the plan's device memory, which the
+ * launch leaves empty again, so runs and replays of one plan must be ordered
+ * (one stream, or events between streams): two overlapping launches of the
+ * same plan would share its queue.  Different plans may run concurrently. */
 int sgemm_grouped_plan_run(sgemm_grouped_plan plan, sgemm_stream_t stream);
 /* Capture the launch into a hipGraph (stream capture on `stream`) ... */
 int sgemm_grouped_plan_capture(sgemm_grouped_plan plan, sgemm_stream_t stream);
