@@ -297,11 +297,21 @@ def cpu_baseline(size: int, cols: int):
         orc.sgemm_basic_cm(s, s, s, 1.5, A1, s, B1, s, 0.5, C1, s)
         times.append(time.perf_counter() - t1)
     t1024 = float(np.median(times))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": flops / dt / 1e12, "unit": "TFLOPS", "cores": orc.max_threads(),
             "kind": "port",
             "sample": f"{cols} of {size} output columns of the {size}^3 problem "
                       f"(M={m}, N={cols}, K={k}), alpha=1.5 beta=0.5, {dt:.1f} s",
-            "tflops_1024": 2.0 * s ** 3 / t1024 / 1e12, "ms_1024_median_of_5": t1024 * 1e3}
+            "tflops_1024": 2.0 * s ** 3 / t1024 / 1e12, "ms_1024_median_of_5": t1024 * 1e3,
+            "host_cpus_visible": os.cpu_count(), "cpu_model": model}
 
 
 def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0):
