@@ -190,22 +190,26 @@ def time_form(fn, steps, warmup, stream):
     return start.elapsed_time(stop) / steps  # ms per call
 
 
-def per_call(fn, calls, stream):
-    """Each call timed on its own (one HIP event pair per call, no cache flush):
-    the median, and the mean of the second half as the reference's harness
-    reports it (benchmark.cu:141-183)."""
+def per_call(fn, calls, stream, flush=None):
+    """Each call timed on its own (one HIP event pair per call): the median,
+    and the mean of the second half as the reference's harness reports it
+    (benchmark.cu:141-183).  `flush`, when given, is run between calls,
+    outside the timed pairs, to empty the L2 and Infinity caches as the
+    reference's harness does (common/helper_cuda.h:70-83)."""
     import numpy as np
     import torch
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(calls)]
     for a, b in evs:
+        if flush is not None:
+            flush()
         a.record(stream)
         fn()
         b.record(stream)
     torch.cuda.synchronize()
     t = [a.elapsed_time(b) for a, b in evs]
     return {"calls": calls, "p50_ms": float(np.median(t)),
-            "mean_second_half_ms": float(np.mean(t[calls // 2:]))}
+            "mean_second_half_ms": float(np.mean(t[calls // 2:])), "cache_flush": flush is not None}
 
 
 def error_vs_fp64(C, ref64, scale64):
@@ -506,8 +510,15 @@ def main():
                        0 if args.x3_only else args.warmup, stream)
     tflops = flops / (ms * 1e-3) / 1e12
     forms = {"f32": {"tflops": tflops, "ms": ms, "sclk_sysfs": clk.summary()}}
+    # 512 MiB written between calls: more than the 8 x 4 MB L2 plus the 256 MB
+    # Infinity Cache, so every flushed call starts with its operands in HBM
+    flush_buf = torch.empty(128 << 20, device=dev, dtype=torch.float32)
+
+    def flush():
+        flush_buf.fill_(0.0)
     if not args.x3_only:
         forms["f32"]["per_call"] = per_call(plain, min(args.steps, 20), stream)
+        forms["f32"]["per_call_flushed"] = per_call(plain, min(args.steps, 10), stream, flush)
 
     if not args.no_bf16x3 and sa.lib().sgemm_config_count(sa.FORM_BF16X3) > 0:
         def split():
@@ -518,7 +529,9 @@ def main():
         t3 = flops / (ms3 * 1e-3) / 1e12
         forms["bf16x3"] = {"tflops": t3, "ms": ms3, "speedup_vs_f32": t3 / tflops,
                            "mfma_tflops_issued": 3 * t3, "sclk_sysfs": clk3.summary(),
-                           "per_call": per_call(split, min(args.steps, 20), stream)}
+                           "per_call": per_call(split, min(args.steps, 20), stream),
+                           "per_call_flushed": per_call(split, min(args.steps, 10), stream,
+                                                        flush)}
 
     if not args.no_error:
         print("bench: error vs FP64", file=sys.stderr, flush=True)
