@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -78,31 +79,29 @@ int heuristic(int form, int grouped, int m, int n, int k) {
         }
         // Measured on MI355X (square sweeps 1024..16384, the sizes whose
         // last round is short, and the odd shapes).  128x128x16 with three
-        // workgroups per CU (768 resident) is the fastest tile from one full
-        // round of 128x128 tiles on.  Its tail-split form (config 9) splits
+        // workgroups per CU (768 resident) is the fastest tile from about
+        // 700 tiles of 128x128 on.  Its tail-split form (config 9) splits
         // k for the tiles of a partial last round where the model of
         // tail_split_count says it pays, and is config 5 elsewhere:
         // 3584^3 104 -> 136, 4608^3 117 -> 133, 5120^3 123 -> 139,
-        // 9216^3 135 -> 141 TFLOP/s.  Between one round and 1024 tiles, a
-        // last round more than a quarter full goes to 128x128 with k-major
-        // A and two per CU (config 4).  Below one round: config 4 when its
-        // last wave is at least half full, else 64x64 tiles with split-K
-        // (config 7), which splits k until the grid holds about four
-        // workgroups per CU (splitk_count).  Short k (<= 256) is
-        // epilogue-bound: 128x64 (config 2).
+        // 9216^3 135 -> 141 TFLOP/s.  Short k (<= 256) is epilogue-bound:
+        // 128x64 (config 2).
         const long long t128 = (long long)div_up(m, 128) * div_up(n, 128);
         if (k <= 256 && t128 >= 512) return 2;
-        if (t128 >= 1024) return 9;
-        if (t128 >= 768) return t128 % 768 <= 192 ? 9 : 4;
-        // Config 5's tile with split-K (config 10) from 300 tiles of 128x128
-        // and between 180 and 255, where the 64x64 tiles leave their last
-        // round short: 1792^3 91 -> 105, 2304^3 100 -> 120, 2560^3 108 ->
-        // 121, 3200^3 119 -> 130 TFLOP/s.  At 256 tiles (2048^3) the 64x64
-        // tiles fill exactly four per CU and win (118 vs 101).
-        if (k >= 1024 && (t128 >= 300 || (t128 >= 180 && t128 < 256))) return 10;
-        if (t128 >= 512) {
-            const long long rem = t128 % 512;
-            return (rem == 0 || rem >= 256) ? 4 : 7;
+        if (t128 >= 700) return 9;
+        // Below that, 64x64 tiles (config 7: loads two k-tiles ahead,
+        // split-K when the grid holds fewer than about four workgroups per
+        // CU) unless their tiles load the CUs unevenly: with t tiles per CU
+        // on average, the busiest CU runs ceil(t).  Under 90 % of that, and
+        // from 180 tiles of 128x128 with k >= 1024, config 5's tile with
+        // split-K (config 10) is faster (square sweep 1024..5120):
+        //   1792^3 (3.1 64x64 tiles per CU) 96 -> 105, 2304^3 (5.1) 108 ->
+        //   119, 2560^3 (6.25) 115 -> 121 TFLOP/s;
+        // while evenly loaded sizes keep 64x64: 2688^3 (6.9) 127 vs 112,
+        // 3072^3 (9.0) 131 vs 120, 3328^3 (10.6) 128 vs 119.
+        if (k >= 1024 && t128 >= 180) {
+            const double t = (double)div_up(m, 64) * div_up(n, 64) / device_cu_count();
+            if (t / std::ceil(t) < 0.9) return 10;
         }
         return 7;
     }

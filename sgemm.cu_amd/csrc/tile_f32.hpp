@@ -38,13 +38,19 @@ namespace sgemm_amd {
 // ds_read_b128 (four MFMA steps) instead of four 4-byte reads.
 // WPE_: waves per SIMD the register allocation must allow.
 // SCHED_: scheduling-hint pattern for full k-tiles (0: none; f32_schedule).
+// PF_: k-tiles whose global loads are in flight at once.  1: the loads of
+// k-tile t+1 are issued while t is multiplied and stored to LDS behind its
+// first k-step.  PF > 1 keeps PF register stages, so the loads of t+PF are
+// issued PF-1 k-tiles before they are stored: for grids with one workgroup
+// per CU, where no other workgroup covers the memory latency.
 template <int BM_, int BN_, int BK_, int WM_, int WN_, bool AT_ = false, int WPE_ = 1,
-          int SCHED_ = 0>
+          int SCHED_ = 0, int PF_ = 1>
 struct F32Tile {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
     static constexpr bool AT = AT_;
     static constexpr int WPE = WPE_;
     static constexpr int SCHED = SCHED_;
+    static constexpr int PF = PF_;
     static constexpr int NT = 64 * WM * WN;
     static constexpr int TM = BM / WM, TN = BN / WN;  // wave tile
     static constexpr int MB = TM / 16, NB = TN / 16;  // 16x16 MFMA blocks per wave
@@ -65,6 +71,7 @@ struct F32Tile {
     static_assert(!AT || (A_VEC % 4 == 0 && (BM / 4) * (BK / 4) == NT * (A_VEC / 4)),
                   "transposed A needs whole 4x4 sub-blocks per thread");
     static_assert(LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
+    static_assert(PF >= 1 && PF <= 4, "prefetch depth");
 };
 
 // Rows r..r+3 of one column, zero outside [0, valid).  VEC: one 16-byte load
@@ -168,23 +175,27 @@ struct F32TileRunner {
         for (int r = 0; r < T::B_VEC; ++r) b_off[r] = 4 * (b_k + (b_n + r * T::B_NSTEP) * P.ldb);
         const long long a_ext = extent(M, K, P.lda), b_ext = extent(K, N, P.ldb);
 
-        // Raw loads land in ra/rb; the k >= K masking happens in sstore, after
-        // the MFMAs of the current tile, so nothing waits on the loads early.
-        f32x4 ra[T::A_VEC], rb[T::B_VEC];
+        // Raw loads land in ra/rb (register stage k-tile % PF); the k >= K
+        // masking happens in sstore, after the MFMAs of the current tile, so
+        // nothing waits on the loads early.  Stage indices are compile-time
+        // constants (std::integral_constant) so the stages stay in registers.
+        f32x4 ra[T::PF][T::A_VEC], rb[T::PF][T::B_VEC];
 
-        auto gload = [&](int k0) {
+        auto gload = [&](int k0, auto stage_tag) {
+            constexpr int q = decltype(stage_tag)::value;
             const long long a_base = m0 + (long long)k0 * P.lda;
             const long long b_base = k0 + (long long)n0 * P.ldb;
             const auto ar = make_rsrc(P.A + a_base, 4 * (a_ext - a_base));
             const auto br = make_rsrc(P.B + b_base, 4 * (b_ext - b_base));
 #pragma unroll
-            for (int r = 0; r < T::A_VEC; ++r) ra[r] = bload4<VEC>(ar, a_off[r]);
+            for (int r = 0; r < T::A_VEC; ++r) ra[q][r] = bload4<VEC>(ar, a_off[r]);
 #pragma unroll
-            for (int r = 0; r < T::B_VEC; ++r) rb[r] = bload4<VEC>(br, b_off[r]);
+            for (int r = 0; r < T::B_VEC; ++r) rb[q][r] = bload4<VEC>(br, b_off[r]);
         };
         // MASK: zero k >= K (only the last k-tile can reach past K).
-        auto sstore_impl = [&](float* stage, int k0, auto mask) {
+        auto sstore_impl = [&](float* stage, int k0, auto mask, auto stage_tag) {
             constexpr bool MASK = decltype(mask)::value;
+            constexpr int q = decltype(stage_tag)::value;
             float* As = stage;
             float* Bs = stage + T::A_FLOATS;
             if constexpr (T::AT) {
@@ -195,7 +206,7 @@ struct F32TileRunner {
                     for (int i = 0; i < 4; ++i) {
                         f32x4 v;  // row 4*m4 + i, k = 4*at_k4 .. +3
 #pragma unroll
-                        for (int kk = 0; kk < 4; ++kk) v[kk] = !MASK || kk < ka ? ra[4 * sb + kk][i] : 0.f;
+                        for (int kk = 0; kk < 4; ++kk) v[kk] = !MASK || kk < ka ? ra[q][4 * sb + kk][i] : 0.f;
                         const int row = 4 * (at_m4 + sb * (T::NT / AS)) + i;
                         *reinterpret_cast<f32x4*>(&As[row * T::SA + 4 * at_k4]) = v;
                     }
@@ -203,7 +214,7 @@ struct F32TileRunner {
 #pragma unroll
                 for (int r = 0; r < T::A_VEC; ++r) {
                     const f32x4 v =
-                        !MASK || (k0 + a_k + r * T::A_KSTEP < K) ? ra[r] : f32x4{0.f, 0.f, 0.f, 0.f};
+                        !MASK || (k0 + a_k + r * T::A_KSTEP < K) ? ra[q][r] : f32x4{0.f, 0.f, 0.f, 0.f};
                     *reinterpret_cast<f32x4*>(&As[(a_k + r * T::A_KSTEP) * T::SA + a_m]) = v;
                 }
             }
@@ -212,14 +223,15 @@ struct F32TileRunner {
             for (int r = 0; r < T::B_VEC; ++r) {
                 f32x4 v;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) v[e] = !MASK || e < kv ? rb[r][e] : 0.f;
+                for (int e = 0; e < 4; ++e) v[e] = !MASK || e < kv ? rb[q][r][e] : 0.f;
                 *reinterpret_cast<f32x4*>(&Bs[(b_n + r * T::B_NSTEP) * T::SB + b_k]) = v;
             }
         };
-        auto sstore = [&](float* stage, int k0) {
-            if (k0 + BK <= K) sstore_impl(stage, k0, std::false_type{});
-            else sstore_impl(stage, k0, std::true_type{});
+        auto sstore = [&](float* stage, int k0, auto stage_tag) {
+            if (k0 + BK <= K) sstore_impl(stage, k0, std::false_type{}, stage_tag);
+            else sstore_impl(stage, k0, std::true_type{}, stage_tag);
         };
+        using Stage0 = std::integral_constant<int, 0>;
 
         f32x4 acc[T::MB][T::NB];
 #pragma unroll
@@ -231,22 +243,39 @@ struct F32TileRunner {
         const int kend = sk ? min(K, kbeg + sk->kchunk) : K;
         const int ktiles = kend > kbeg ? div_up(kend - kbeg, BK) : 0;
         if (ktiles > 0) {
-            gload(kbeg);
-            sstore(smem, kbeg);
+            gload(kbeg, Stage0{});
+            if constexpr (T::PF > 1) {
+#pragma unroll
+                for (int q = 1; q < T::PF; ++q) {
+                    if (q < ktiles) {
+                        // q is a loop constant after unrolling; dispatch it
+                        // to the matching compile-time stage.
+                        if (q == 1) gload(kbeg + BK, std::integral_constant<int, 1>{});
+                        if constexpr (T::PF > 2)
+                            if (q == 2) gload(kbeg + 2 * BK, std::integral_constant<int, 2 % T::PF>{});
+                        if constexpr (T::PF > 3)
+                            if (q == 3) gload(kbeg + 3 * BK, std::integral_constant<int, 3 % T::PF>{});
+                    }
+                }
+            }
+            sstore(smem, kbeg, Stage0{});
             __syncthreads();
         }
         // One k-tile: MFMAs out of the current stage while the next k-tile
-        // is loaded and stored to the other one.  NEXT_FULL: a next k-tile
-        // exists and lies entirely below K, so the body has no masking and
-        // no branch.
-        auto iteration = [&](int kt, auto next_full_tag) {
+        // is stored to the other one and k-tile kt+PF is loaded into the
+        // register stage k-tile kt has just vacated.  NEXT_FULL: a next
+        // k-tile exists and lies entirely below K, so the body has no
+        // masking (and, at PF 1, no branch).  P = kt % PF.
+        auto iteration = [&](int kt, auto next_full_tag, auto p_tag) {
             constexpr bool NEXT_FULL = decltype(next_full_tag)::value;
+            constexpr int P = decltype(p_tag)::value;
+            using NextStage = std::integral_constant<int, (P + 1) % T::PF>;
             const float* As = smem + (kt & 1) * T::STAGE;
             const float* Bs = As + T::A_FLOATS;
             float* next_stage = smem + ((kt + 1) & 1) * T::STAGE;
             const bool more = NEXT_FULL || kt + 1 < ktiles;
             const int knext = kbeg + (kt + 1) * BK;
-            if (more) gload(knext);
+            if (T::PF == 1 ? more : kt + T::PF < ktiles) gload(kbeg + (kt + T::PF) * BK, p_tag);
             if constexpr (NEXT_FULL && T::SCHED == 0) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int kk = 0; kk < BK; kk += 16) {
@@ -283,10 +312,10 @@ struct F32TileRunner {
                 if (kk == 0) {
                     if constexpr (NEXT_FULL) {
                         if constexpr (T::SCHED == 0) __builtin_amdgcn_sched_barrier(0);
-                        sstore_impl(next_stage, knext, std::false_type{});
+                        sstore_impl(next_stage, knext, std::false_type{}, NextStage{});
                         if constexpr (T::SCHED == 0) __builtin_amdgcn_sched_barrier(0);
                     } else if (more) {
-                        sstore(next_stage, knext);
+                        sstore(next_stage, knext, NextStage{});
                     }
                 }
             }
@@ -299,13 +328,33 @@ struct F32TileRunner {
         const int nfast = min(ktiles, kfull) - 1;
         int kt = 0;
         if constexpr (PEEL) {
-            for (; kt < nfast; ++kt) {
-                iteration(kt, std::true_type{});
+            // PF k-tiles per pass, so each one's register stage is a constant.
+            for (; kt + T::PF <= nfast; kt += T::PF) {
+                iteration(kt, std::true_type{}, Stage0{});
                 __syncthreads();
+                if constexpr (T::PF > 1) {
+                    iteration(kt + 1, std::true_type{}, std::integral_constant<int, 1>{});
+                    __syncthreads();
+                }
+                if constexpr (T::PF > 2) {
+                    iteration(kt + 2, std::true_type{}, std::integral_constant<int, 2 % T::PF>{});
+                    __syncthreads();
+                }
+                if constexpr (T::PF > 3) {
+                    iteration(kt + 3, std::true_type{}, std::integral_constant<int, 3 % T::PF>{});
+                    __syncthreads();
+                }
             }
         }
         for (; kt < ktiles; ++kt) {
-            iteration(kt, std::false_type{});
+            const int p = kt % T::PF;
+            if (p == 0) iteration(kt, std::false_type{}, Stage0{});
+            if constexpr (T::PF > 1)
+                if (p == 1) iteration(kt, std::false_type{}, std::integral_constant<int, 1>{});
+            if constexpr (T::PF > 2)
+                if (p == 2) iteration(kt, std::false_type{}, std::integral_constant<int, 2 % T::PF>{});
+            if constexpr (T::PF > 3)
+                if (p == 3) iteration(kt, std::false_type{}, std::integral_constant<int, 3 % T::PF>{});
             __syncthreads();
         }
 
@@ -380,7 +429,9 @@ struct F32TileRunner {
 // ---------------------------------------------------------------- configs
 // Config ids of the plain FP32 form (the order is the ABI's config id).
 using F32_128x128 = F32Tile<128, 128, 32, 2, 2>;
-using F32_64x64 = F32Tile<64, 64, 32, 2, 2>;
+// 64x64 runs one workgroup per CU on grids of up to 256 tiles, so its loads
+// are issued two k-tiles ahead: 2048^3 118 -> 125, 3072^3 122 -> 131 TFLOP/s.
+using F32_64x64 = F32Tile<64, 64, 32, 2, 2, false, 1, 0, 2>;
 using F32_128x64 = F32Tile<128, 64, 32, 2, 2>;
 using F32_256x128 = F32Tile<256, 128, 16, 2, 2>;
 using F32_128x128_at = F32Tile<128, 128, 32, 2, 2, true, 2>;
