@@ -119,14 +119,18 @@ int heuristic(int form, int grouped, int m, int n, int k) {
     // Config 9 is config 2 with a tail split: from one full round on, the
     // tiles of a partial last round are cut into k-ranges, which costs what
     // tail_split_count models instead of a whole round; when it would not
-    // split it is config 2 and is left to config 2.
-    enum { kWhole = 0, kSplitK = 1, kTailSplit = 2 };
+    // split it is config 2 and is left to config 2.  Config 11 is config 2
+    // with stream-K over those tiles (or over all tiles, short of a full
+    // round), costing what streamk_iters models: 3072^3 244 -> 314, 5120^3
+    // 352 -> 387 TFLOP/s.
+    enum { kWhole = 0, kSplitK = 1, kTailSplit = 2, kStreamK = 3 };
     struct Cand { int id, bm, bn, bk, per_cu; double rate; int split; };
     static const Cand cands[] = {
         {2, 256, 256, 32, 1, 1.00, kWhole},   {9, 256, 256, 32, 1, 1.00, kTailSplit},
         {5, 256, 128, 16, 2, 0.92, kWhole},   {8, 128, 128, 32, 2, 0.84, kWhole},
         {3, 256, 128, 32, 1, 0.69, kWhole},   {7, 64, 64, 64, 2, 0.47, kSplitK},
         {10, 128, 128, 32, 2, 0.70, kSplitK},  // config 8's rate less its split's partials
+        {11, 256, 256, 32, 1, 1.00, kStreamK},
     };
     const long long cus = device_cu_count();
     int best = 7;
@@ -142,6 +146,19 @@ int heuristic(int form, int grouped, int m, int n, int k) {
             if (s <= 1) continue;
             const double round = (double)c.per_cu * c.bm * c.bn / c.rate;
             cost = (double)(tiles / slots) * round + tail * round;
+        } else if (c.split == kStreamK) {
+            double part = 1.0;
+            if (tiles % slots == 0 ||
+                streamk_iters(tiles % slots, slots, div_up(k, c.bk), kX3TileSplitBytes,
+                              kX3StreamKPieceIters, &part) <= 0)
+                continue;
+            // The stream-K part replaces a round of whole tiles; take it only
+            // where the model saves at least 5 % of that round (3584^3, 196
+            // tiles: modelled 4.5 % faster than whole tiles, measured 2 %
+            // slower).
+            if (part > 0.95) continue;
+            const double round = (double)c.per_cu * c.bm * c.bn / c.rate;
+            cost = (double)(tiles / slots) * round + part * round;
         } else {
             const long long splits =
                 c.split == kSplitK

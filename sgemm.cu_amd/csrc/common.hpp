@@ -49,10 +49,26 @@ struct SplitK {
     int* counters;    // [tiles], zero between launches
     int splits;
     int kchunk;       // multiple of the tile's BK
-    // Tail split (FP32 kernel only): workgroups [0, dp_tiles) compute whole
-    // tiles [0, dp_tiles) unsplit; the workgroups after them split the
-    // remaining tiles' k ranges.  0: every workgroup is a split unit.
+    // Tail split: workgroups [0, dp_tiles) compute whole tiles [0, dp_tiles)
+    // unsplit; the workgroups after them split the remaining tiles' k
+    // ranges.  0: every workgroup is a split unit.
     int dp_tiles;
+    // Stream-K over the tiles after dp_tiles (iters > 0): workgroup g of the
+    // stream-K part runs k-tile iterations [g * iters, (g + 1) * iters) of
+    // those tiles' iteration space (tile-major, k-tiles in order), so a tile
+    // is cut into pieces at workgroup boundaries.  A cut tile's partials and
+    // counter sit at the index of the workgroup holding its last k-tile,
+    // `pieces` partial slots apart.
+    int iters = 0;
+    int pieces = 0;
+};
+
+// The k range one workgroup accumulates for a tile: k in [kbeg, kend), piece
+// q of the tile's n pieces (in k order).  n > 1: the partials are reduced
+// through workspace region `region` of `stride` piece slots, and counter
+// `region`.
+struct KPiece {
+    int kbeg, kend, q, n, region, stride;
 };
 
 // Cache policy of the partials' loads and stores: sc1 (agent coherence).
@@ -66,9 +82,9 @@ __device__ inline f32x4 load_agent(__amdgpu_buffer_rsrc_t r, int off) {
 }
 
 // Called by every thread of the workgroup after its partial stores.  Returns
-// true in the workgroup that arrives last for `tile` (and resets the counter
-// for the next launch).
-__device__ inline bool splitk_arrive(const SplitK& sk, int tile, int* flag_lds) {
+// true in the workgroup that arrives last of the n for `tile` (and resets the
+// counter for the next launch).
+__device__ inline bool splitk_arrive(const SplitK& sk, int tile, int* flag_lds, int n) {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);  // keep the stores above the wait
     __builtin_amdgcn_s_waitcnt(0);            // this wave's partial stores completed
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -77,7 +93,7 @@ __device__ inline bool splitk_arrive(const SplitK& sk, int tile, int* flag_lds) 
         *flag_lds = __hip_atomic_fetch_add(&sk.counters[tile], 1, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    const bool last = *flag_lds == sk.splits - 1;
+    const bool last = *flag_lds == n - 1;
     if (last && threadIdx.x == 0)
         __hip_atomic_store(&sk.counters[tile], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return last;
@@ -86,6 +102,9 @@ __device__ inline bool splitk_arrive(const SplitK& sk, int tile, int* flag_lds) 
 static_assert(sizeof(Problem) % 8 == 0, "descriptor packing");
 
 __host__ __device__ constexpr int div_up(int a, int b) { return (a + b - 1) / b; }
+__host__ __device__ constexpr long long div_up_ll(long long a, long long b) {
+    return (a + b - 1) / b;
+}
 
 // Elements a column-major (rows x cols, ld) operand spans.
 __host__ __device__ inline long long extent(int rows, int cols, int ld) {

@@ -13,7 +13,8 @@ struct ConfigInfo {
     int wm, wn;      // waves along m and n
     const char* name;
     int splitk;      // 1: the k range is split across workgroups (splitk_count);
-                     // 2: only the tiles of a partial last round are split
+                     // 2: only the tiles of a partial last round are split;
+                     // 3: stream-K over the tiles of a partial last round
 };
 
 // Compute units of the current device (queried once per process).
@@ -48,6 +49,21 @@ constexpr size_t kX3TileSplitBytes = 256 * 256 * 4;
 // 8 with split-K): 1536^3 116 -> 179, 2048^3 218 -> 241 TFLOP/s, while four
 // k-tiles per split ran 1024^3 at half the 64x64 split config's speed.
 constexpr int kX3WideSplitKtiles = 16;
+// Stream-K (BF16x3 config 11): the cost of one piece of a cut tile, in
+// k-tile iterations of the 256x256 tile -- writing its partials and, for the
+// last piece, reading every piece back.  Fitted to 3072^3 (three pieces per
+// tile) and 3584^3 (fifteen).
+constexpr double kX3StreamKPieceIters = 7.0;
+
+// Stream-K over the `rem` tiles of a partial round (or all tiles, without a
+// full round) of `iters` k-tiles each, spread over `slots` workgroups:
+// returns the k-tiles per workgroup w that minimises w + kPieceIters x
+// (pieces of the most-cut tile, (iters - 1) / w + 2), at least the even share
+// and at most 16 pieces and 512 MiB of partials (pieces x split_bytes per
+// workgroup).  *part: the stream-K part's time in rounds of whole tiles.
+// 0 when no w qualifies.
+long long streamk_iters(long long rem, long long slots, long long iters, size_t split_bytes,
+                        double piece_iters, double* part);
 // Plain FP32 rate of one CU running config 5's tile three at a time (141
 // TFLOP/s over 256 CUs at 8192^3).
 constexpr double kF32FlopsPerCuUs = 0.55e6;

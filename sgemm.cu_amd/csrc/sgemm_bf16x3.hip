@@ -31,11 +31,55 @@ void k_sgemm_bf16x3(Problem P, SplitK sk) {
     }
 }
 
+// Stream-K over the tail: workgroups [0, sk.dp_tiles) compute tiles [0,
+// sk.dp_tiles) whole (every full round of resident workgroups); workgroup
+// dp_tiles + g of the rest runs k-tile iterations [g * iters, (g + 1) *
+// iters) of the remaining tiles, taken tile-major with k-tiles in order.  A
+// tile wholly inside one workgroup's range is stored straight to C; a tile
+// cut at a workgroup boundary is reduced from its pieces in k order by the
+// last of them to finish.
+template <class T, bool VEC>
+__global__ __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(T::WPE)))
+void k_sgemm_bf16x3_sk(Problem P, SplitK sk) {
+    __shared__ __attribute__((aligned(16))) char smem[T::LDS_BYTES];
+    int tm, tn;
+    const int b = blockIdx.x;
+    if (b < sk.dp_tiles) {
+        tile_coords(xcd_remap(b, sk.dp_tiles), P.tiles_m, P.tiles_n, kX3GroupN, tm, tn);
+        X3TileRunner<T, VEC>::run(P, tm, tn, smem);
+        return;
+    }
+    const int nwg = (int)gridDim.x - sk.dp_tiles;
+    const long long g = xcd_remap(b - sk.dp_tiles, nwg);
+    const int I = div_up(P.k, T::BK);  // k-tiles per tile
+    const long long L = ((long long)P.tiles_m * P.tiles_n - sk.dp_tiles) * I;
+    const long long w = sk.iters;
+    long long x = g * w;
+    const long long end = min(L, x + w);
+    while (x < end) {
+        const int t = (int)(x / I);
+        const int kt0 = (int)(x - (long long)t * I);
+        const int kt1 = (int)min((long long)I, kt0 + (end - x));
+        const int first = (int)((long long)t * I / w);          // workgroup of k-tile 0
+        const int last = (int)(((long long)t * I + I - 1) / w);  // ... and of the last
+        tile_coords(sk.dp_tiles + t, P.tiles_m, P.tiles_n, kX3GroupN, tm, tn);
+        X3TileRunner<T, VEC>::run_piece(
+            P, tm, tn, smem, &sk,
+            KPiece{kt0 * T::BK, min(P.k, kt1 * T::BK), (int)g - first, last - first + 1, last,
+                   sk.pieces});
+        x = (long long)t * I + kt1;
+        // The next piece's LDS stores must not overtake another wave's read
+        // of the arrival flag in LDS.
+        __syncthreads();
+    }
+}
+
 // Ids 6-7 run the tiles of ids 0-1 with split-K (splitk = 1); id 8 is a
 // 128x128 tile with one accumulator set, two workgroups per CU; id 9 runs
 // the tile of id 2 with a tail split (splitk = 2, as FP32 config 9); id 10
 // runs the tile of id 8 with split-K, each split at least 16 k-tiles deep
-// (shorter splits of this tile lose to the 64x64 one).
+// (shorter splits of this tile lose to the 64x64 one).  Id 11 runs the tile
+// of id 2 with stream-K over the tiles of a partial last round (splitk = 3).
 static const ConfigInfo kBf16x3Configs[] = {
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m16_dual"},
     {64, 64, 64, 2, 2, "bf16x3_64x64x64_w2x2_m16_dual"},
@@ -48,6 +92,7 @@ static const ConfigInfo kBf16x3Configs[] = {
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m16_single_2cu"},
     {256, 256, 32, 2, 4, "bf16x3_256x256x32_w2x4_m16_single_tailsplit", 2},
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m16_single_2cu_splitk", 1},
+    {256, 256, 32, 2, 4, "bf16x3_256x256x32_w2x4_m16_single_streamk", 3},
 };
 
 int bf16x3_config_count() { return (int)(sizeof(kBf16x3Configs) / sizeof(kBf16x3Configs[0])); }
@@ -152,6 +197,59 @@ static hipError_t launch_x3_tail(const Problem& p, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Stream-K over the tail: the r = tiles % slots tiles after the full rounds
+// (all tiles when there is no full round) are spread, k-tile by k-tile, over
+// up to `slots` workgroups of w k-tiles each (streamk_iters: w trades the
+// longer ranges against fewer pieces per cut tile).  Falls back to one launch of whole tiles when
+// there is no partial round, or when the stream is being captured and its
+// workspace would have to grow.
+template <class T>
+static hipError_t launch_x3_streamk(const Problem& p, hipStream_t s) {
+    Problem q = p;
+    q.tiles_m = div_up(p.m, T::BM);
+    q.tiles_n = div_up(p.n, T::BN);
+    q.tile_begin = 0;
+    const long long tiles = (long long)q.tiles_m * q.tiles_n;
+    if (tiles <= 0) return hipSuccess;
+    if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
+    const long long slots = x3_resident_workgroups<T>();
+    const long long rem = tiles % slots;
+    constexpr size_t acc_bytes = T::MF == 16 ? 16 : 64;
+    const size_t split_bytes = (size_t)T::MB * T::NB * acc_bytes * T::NT;
+    const long long I = div_up(p.k, T::BK);
+    SplitK sk{nullptr, nullptr, 1, p.k, 0};
+    long long nwg = 0;
+    if (rem > 0 && p.k > 0) {
+        const long long L = rem * I;
+        double part = 1.0;
+        const long long w = streamk_iters(rem, slots, I, split_bytes, kX3StreamKPieceIters, &part);
+        nwg = w > 0 ? div_up_ll(L, w) : 0;
+        sk.iters = (int)w;
+        sk.pieces = w > 0 ? (int)((I - 1) / w + 2) : 0;
+        if (nwg > 1) {
+            const hipError_t e = splitk_workspace(s, (size_t)nwg * sk.pieces * split_bytes,
+                                                  (int)nwg, &sk.partials, &sk.counters);
+            if (e == hipErrorStreamCaptureUnsupported) nwg = 0;
+            else if (e != hipSuccess) return e;
+        } else {
+            nwg = 0;
+        }
+    }
+    (void)hipGetLastError();
+    const bool vec = vec4_ok(p);
+    if (nwg > 0) {
+        sk.dp_tiles = (int)(tiles - rem);
+        const dim3 grid((unsigned)(tiles - rem + nwg));
+        if (vec) hipLaunchKernelGGL((k_sgemm_bf16x3_sk<T, true>), grid, dim3(T::NT), 0, s, q, sk);
+        else hipLaunchKernelGGL((k_sgemm_bf16x3_sk<T, false>), grid, dim3(T::NT), 0, s, q, sk);
+    } else {
+        const dim3 grid((unsigned)tiles);
+        if (vec) hipLaunchKernelGGL((k_sgemm_bf16x3<T, true, false>), grid, dim3(T::NT), 0, s, q, sk);
+        else hipLaunchKernelGGL((k_sgemm_bf16x3<T, false, false>), grid, dim3(T::NT), 0, s, q, sk);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_bf16x3(int id, const Problem& p, hipStream_t s) {
     if (id < 0 || id >= bf16x3_config_count()) return hipErrorInvalidValue;
     switch (id) {
@@ -159,6 +257,7 @@ hipError_t launch_bf16x3(int id, const Problem& p, hipStream_t s) {
         case 7: return launch_x3<X3_64x64, true>(p, s);
         case 9: return launch_x3_tail<X3_256x256_w8>(p, s);
         case 10: return launch_x3<X3_128x128_2cu, true, kX3WideSplitKtiles>(p, s);
+        case 11: return launch_x3_streamk<X3_256x256_w8>(p, s);
         default: break;
     }
     hipError_t e = hipErrorInvalidValue;

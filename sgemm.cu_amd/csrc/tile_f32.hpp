@@ -373,7 +373,7 @@ struct F32TileRunner {
                 for (int j = 0; j < T::NB; ++j)
                     store_agent(part, split * SPLIT_BYTES + ((i * T::NB + j) * T::NT + t) * 16,
                                 acc[i][j]);
-            if (!splitk_arrive(*sk, tile, reinterpret_cast<int*>(smem))) return;
+            if (!splitk_arrive(*sk, tile, reinterpret_cast<int*>(smem), sk->splits)) return;
             // Sum the partials in split order, this workgroup's own re-read
             // like the others (bit-identical to its registers), so the sum
             // reuses the accumulators' registers.  One split's loads are all

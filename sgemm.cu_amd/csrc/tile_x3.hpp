@@ -120,6 +120,19 @@ struct X3TileRunner {
     // tile's partials and arrival counter.
     __device__ static void run(const Problem& P, int tm, int tn, char* smem,
                                const SplitK* sk = nullptr, int split = 0, int slot = 0) {
+        if (!sk) {
+            run_piece(P, tm, tn, smem, nullptr, KPiece{0, P.k, 0, 1, 0, 1});
+            return;
+        }
+        const int kbeg = split * sk->kchunk;
+        run_piece(P, tm, tn, smem, sk,
+                  KPiece{kbeg, min(P.k, kbeg + sk->kchunk), split, sk->splits, slot, sk->splits});
+    }
+
+    // As run, for the k range and reduction slot of `pc` (pc.n == 1: the
+    // whole tile's k range, stored straight to C).
+    __device__ static void run_piece(const Problem& P, int tm, int tn, char* smem,
+                                     const SplitK* sk, const KPiece pc) {
         const int t = threadIdx.x;
         const int lane = t & 63, wave = t >> 6;
         const int wm = wave % T::WM, wn = wave / T::WM;
@@ -218,8 +231,8 @@ struct X3TileRunner {
         // consecutive k at kk + 8 * (lane / MF).
         const int lrow = lane % T::MF, lk = 8 * (lane / T::MF);
 
-        const int kbeg = sk ? split * sk->kchunk : 0;
-        const int kend = sk ? min(K, kbeg + sk->kchunk) : K;
+        const int kbeg = pc.kbeg;
+        const int kend = pc.kend;
         const int ktiles = kend > kbeg ? div_up(kend - kbeg, BK) : 0;
         if (ktiles > 0) {
             gload(kbeg);
@@ -318,10 +331,10 @@ struct X3TileRunner {
             __syncthreads();
         }
 
-        if (sk) {
-            // Fold the cross-term accumulators in, dump this split's sums in
+        if (sk && pc.n > 1) {
+            // Fold the cross-term accumulators in, dump this piece's sums in
             // lane order, and let the last workgroup of the tile add all
-            // splits in split order.
+            // pieces in k order.
             if constexpr (T::DUAL) {
 #pragma unroll
                 for (int i = 0; i < T::MB; ++i)
@@ -334,10 +347,10 @@ struct X3TileRunner {
             constexpr int NV = sizeof(Acc) / 16;  // float4 pieces per accumulator
             constexpr int NACC = T::MB * T::NB * NV;
             constexpr int SPLIT_BYTES = NACC * T::NT * 16;
-            const int tile = slot;
+            const int tile = pc.region;
             const __amdgpu_buffer_rsrc_t part =
-                make_rsrc(sk->partials + (size_t)tile * sk->splits * NACC * T::NT * 4,
-                          (long long)sk->splits * SPLIT_BYTES);
+                make_rsrc(sk->partials + (size_t)tile * pc.stride * NACC * T::NT * 4,
+                          (long long)pc.stride * SPLIT_BYTES);
             auto piece = [&](int i, int j, int v) -> f32x4 {
                 return f32x4{acc[i][j][4 * v], acc[i][j][4 * v + 1], acc[i][j][4 * v + 2],
                              acc[i][j][4 * v + 3]};
@@ -349,8 +362,8 @@ struct X3TileRunner {
                 for (int j = 0; j < T::NB; ++j)
 #pragma unroll
                     for (int v = 0; v < NV; ++v)
-                        store_agent(part, split * SPLIT_BYTES + off(i, j, v), piece(i, j, v));
-            if (!splitk_arrive(*sk, tile, reinterpret_cast<int*>(smem))) return;
+                        store_agent(part, pc.q * SPLIT_BYTES + off(i, j, v), piece(i, j, v));
+            if (!splitk_arrive(*sk, tile, reinterpret_cast<int*>(smem), pc.n)) return;
             // This workgroup's own partial is re-read like the others
             // (bit-identical to its registers), so the sum reuses the
             // accumulators' registers.
@@ -364,7 +377,7 @@ struct X3TileRunner {
 #pragma unroll
                         for (int r = 0; r < 4; ++r) acc[i][j][4 * v + r] = x[r];
                     }
-            for (int q = 1; q < sk->splits; ++q) {
+            for (int q = 1; q < pc.n; ++q) {
 #pragma unroll
                 for (int i = 0; i < T::MB; ++i)
 #pragma unroll

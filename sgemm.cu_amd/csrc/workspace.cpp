@@ -91,6 +91,28 @@ int tail_split_count(long long tiles, long long slots, int per_cu, int k, int bk
     return best;
 }
 
+long long streamk_iters(long long rem, long long slots, long long iters, size_t split_bytes,
+                        double piece_iters, double* part) {
+    *part = 1.0;
+    if (rem <= 0 || slots <= 0 || iters <= 0) return 0;
+    const long long L = rem * iters;
+    const long long w0 = std::max(1LL, (L + slots - 1) / slots);
+    long long best = 0;
+    double best_t = 0;
+    for (long long w = w0; w <= std::max(w0, iters); ++w) {
+        const long long pieces = (iters - 1) / w + 2;
+        if (pieces > 16) continue;
+        if ((double)((L + w - 1) / w) * pieces * split_bytes > 512.0 * 1024 * 1024) continue;
+        const double t = (double)w + piece_iters * pieces;
+        if (best == 0 || t < best_t) {
+            best = w;
+            best_t = t;
+        }
+    }
+    if (best > 0) *part = best_t / iters;
+    return best;
+}
+
 hipError_t splitk_plan(hipStream_t stream, long long tiles, int k, int bk, size_t split_bytes,
                        SplitK* sk, bool f32, int min_ktiles) {
     sk->partials = nullptr;

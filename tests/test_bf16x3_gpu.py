@@ -22,7 +22,7 @@ from gpu_util import (Case, assert_ok, check, compact, download, log, padding_un
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gemm_cases.json")
-N_X3_CONFIGS = 11
+N_X3_CONFIGS = 12
 
 
 def test_config_table():
@@ -98,7 +98,8 @@ def test_beta_zero_nan_c_and_k_zero():
 
 
 @pytest.mark.parametrize("config,m,n,k", [(6, 190, 77, 4096), (7, 190, 77, 4096),
-                                           (10, 190, 77, 4096), (9, 4352, 4096, 1024)])
+                                           (10, 190, 77, 4096), (9, 4352, 4096, 1024),
+                                           (11, 3000, 2000, 1024), (11, 4352, 4096, 1024)])
 def test_beta_zero_nan_c_with_split_k(config, m, n, k):
     """beta == 0 never reads C on the split paths (split-K and the tail
     split's reduction run the same epilogue)."""
@@ -263,3 +264,83 @@ def test_tail_split_graph_replays_and_capture_fallback():
         g2.replay()
         torch.cuda.synchronize()
         assert np.array_equal(download(dC), unsplit), rep
+
+
+# ------------------------------------------------------------ stream-K
+X3_STREAMK_CONFIG = 11  # config 2's 256x256 tile; stream-K over a partial round's tiles
+
+
+@pytest.mark.parametrize("m,n,k,ld_pad", [
+    (1000, 1100, 2000, 3),   # 20 tiles, no full round: every tile cut into many pieces
+    (3000, 3100, 1500, 0),   # 156 tiles
+    (4352, 4096, 1024, 0),   # 272 tiles: 16 past one 256-CU round
+    (5120, 5120, 640, 1),    # 400 tiles: 144 past one round
+])
+def test_streamk_deterministic_and_in_tolerance(m, n, k, ld_pad):
+    """In the BF16x3 tolerance against FP64 (every entry below 2^24 of them,
+    sampled above), bit-identical on repeat with another problem of the same
+    shape in between (the pieces' partial slots and counters are reused)."""
+    c = Case(m, n, k, lda=m + ld_pad, ldb=k + ld_pad, ldc=m + ld_pad, seed=m + 2 * n + k,
+             alpha=1.25, beta=0.5)
+    first = run_form(c, sa.FORM_BF16X3, X3_STREAMK_CONFIG)
+    if m * n <= (1 << 21):
+        assert_ok(check(c, first, orc.bf16x3_tol_mult(k)), f"stream-K {(m, n, k)}")
+    else:
+        _sampled_check_x3(c, first, n_samples=4096, seed=k)
+    assert padding_untouched(first, c.C0, m, n, c.ldc)
+    other = Case(m, n, k, lda=m + ld_pad, ldb=k + ld_pad, ldc=m + ld_pad, kind="wide",
+                 seed=m + n + k + 11, alpha=1.0, beta=0.0)
+    out = run_form(other, sa.FORM_BF16X3, X3_STREAMK_CONFIG)
+    _sampled_check_x3(other, out, n_samples=2048, seed=k + 3)
+    again = run_form(c, sa.FORM_BF16X3, X3_STREAMK_CONFIG)
+    assert np.array_equal(first, again), (m, n, k)
+
+
+@pytest.mark.parametrize("m,n,k", [(2560, 2304, 1536), (4352, 4096, 2048)])
+def test_streamk_integer_kat_equals_unsplit(m, n, k):
+    """Integer inputs are BF16-exact and every partial sum is an exact
+    integer, so stream-K equals config 2 (whole tiles) bit for bit."""
+    c = Case(m, n, k, kind="int", seed=23, alpha=1.5, beta=0.5)
+    a = run_form(c, sa.FORM_BF16X3, 2)
+    b = run_form(c, sa.FORM_BF16X3, X3_STREAMK_CONFIG)
+    assert np.array_equal(a, b)
+
+
+def test_streamk_graph_replays_and_capture_fallback():
+    """Captured after a warm-up call on the stream, stream-K replays
+    bit-identically to its eager result; captured on a fresh stream (no
+    workspace yet) it runs config 2's kernel on whole tiles, bit for bit."""
+    import torch
+    m, n, k = 3000, 3100, 1536
+    c = Case(m, n, k, seed=79, alpha=1.0, beta=0.0)
+    dA, dB, dC = c.device()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        sa.run_config(1, X3_STREAMK_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, s)
+        s.synchronize()
+        eager = download(dC).copy()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            sa.run_config(1, X3_STREAMK_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m,
+                          torch.cuda.current_stream())
+    for rep in range(3):
+        dC.fill_(5.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(download(dC), eager), rep
+    sa.run_config(1, 2, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m)
+    torch.cuda.synchronize()
+    whole = download(dC).copy()
+    assert not np.array_equal(whole, eager)  # the tiles were cut
+    fresh = torch.cuda.Stream()
+    g2 = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g2, stream=fresh):
+        sa.run_config(1, X3_STREAMK_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m,
+                      torch.cuda.current_stream())
+    for rep in range(2):
+        dC.fill_(-2.0)
+        torch.cuda.synchronize()
+        g2.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(download(dC), whole), rep
