@@ -71,7 +71,7 @@ def parse():
     return args
 
 
-def sweep(sa, torch, steps, warmup):
+def sweep(sa, torch, warmup):
     """TFLOPS and fraction of peak per size for both forms (one JSON line)."""
     shapes = [(s, s, s) for s in range(1024, 16385, 1024)]
     shapes += [(1000, 1000, 1000), (4097, 3071, 2049), (8192, 8192, 127)]
@@ -82,12 +82,16 @@ def sweep(sa, torch, steps, warmup):
         B = torch.rand(k * n, device="cuda") * 2 - 1
         C = torch.rand(m * n, device="cuda") * 2 - 1
         flops = 2.0 * m * n * k
-        it = max(3, min(steps, int(2e13 / flops)))
+        # About 20 TFLOP of timed work per size (at least 3 calls, at most
+        # 500), after a quarter as many warm-up calls: 20 calls of a small
+        # size finish before the engine clock has risen.
+        it = max(3, min(500, int(2e13 / flops)))
+        warm = max(warmup, it // 4)
         row = {"m": m, "n": n, "k": k}
         for form, name, fn, peak in ((sa.FORM_F32, "f32", sa.sgemm_ex, PEAK_FP32_TFLOPS),
                                      (sa.FORM_BF16X3, "bf16x3", sa.sgemm_bf16x3,
                                       PEAK_BF16_TFLOPS / 3)):
-            ms = time_form(lambda: fn(m, n, k, 1.5, A, m, B, k, 0.5, C, m, stream), it, warmup,
+            ms = time_form(lambda: fn(m, n, k, 1.5, A, m, B, k, 0.5, C, m, stream), it, warm,
                            stream)
             tf = flops / (ms * 1e-3) / 1e12
             cfg = sa.autotune_lookup(form, m, n, k)[0]
@@ -498,7 +502,7 @@ def main():
         return
     if args.sweep:
         sa.lib()
-        sweep(sa, torch, args.steps, args.warmup)
+        sweep(sa, torch, args.warmup)
         return
     sa.lib()
     torch.manual_seed(0)
