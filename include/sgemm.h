@@ -105,7 +105,12 @@ int sgemm_ex(int m, int n, int k, const float* alpha, const float* A, int lda, c
 
 /* Split-precision form: each FP32 input x is split on the GPU into
  * hi = bf16_rne(x), lo = bf16_rne(x - hi); A*B is accumulated in FP32 as
- * hi*hi + hi*lo + lo*hi on BF16 MFMA.  Same operand conventions as sgemm_ex. */
+ * hi*hi + hi*lo + lo*hi on BF16 MFMA.  Same operand conventions as sgemm_ex.
+ * Besides split-K and tail-split configs this form has a stream-K config
+ * (the tiles of a partial round spread k-tile by k-tile over the CUs, cut
+ * tiles summed from their pieces in k order, so also deterministic), with the
+ * same per-stream workspace and capture behaviour; its partials take up to
+ * 512 MiB. */
 int sgemm_bf16x3(int m, int n, int k, const float* alpha, const float* A, int lda,
                  const float* B, int ldb, const float* beta, float* C, int ldc,
                  sgemm_stream_t stream);
