@@ -3,7 +3,7 @@
 //
 //   ./bin/sgemm_benchmark [--mmin=1024] [--mstep=512] [--npts=21] [--wmsize=4096]
 //                         [--wniter=500] [--fname=sgemm.cu] [--savedir=benchmark_results]
-//                         [--form=f32|bf16x3|rocblas]
+//                         [--form=f32|bf16x3|rocblas] [--config=ID]
 //
 // Same protocol as the reference: warm-up of `wniter` calls at wmsize^3, then
 // for each size s = mmin + i*mstep (i < npts): n_iter = max(1000 *
@@ -14,7 +14,8 @@
 // the reference's plot script reads (plot_benchmark_results.py:20-30).
 // --form=rocblas times rocblas_sgemm on the same inputs and names the file
 // rocBLAS.txt, as the reference's CUBLAS=1 build names cuBLAS.txt
-// (benchmark.cu:35-45,150-164).
+// (benchmark.cu:35-45,150-164).  --config=ID runs kernel config ID of the
+// form (sgemm_run_config) instead of the one the library picks.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
@@ -34,9 +35,10 @@ namespace {
 
 struct Runner {
     std::string form;
+    int config;
     rocblas_handle handle = nullptr;
 
-    explicit Runner(const std::string& f) : form(f) {
+    Runner(const std::string& f, int cfg) : form(f), config(cfg) {
         if (form == "rocblas" && rocblas_create_handle(&handle) != rocblas_status_success) {
             std::fprintf(stderr, "rocblas_create_handle failed\n");
             std::exit(EXIT_FAILURE);
@@ -47,7 +49,10 @@ struct Runner {
     }
     void operator()(int s, const float* alpha, float* A, float* B, const float* beta, float* C) {
         int st = SGEMM_STATUS_SUCCESS;
-        if (form == "bf16x3") {
+        if (config >= 0 && form != "rocblas") {
+            st = sgemm_run_config(form == "bf16x3" ? SGEMM_FORM_BF16X3 : SGEMM_FORM_F32, config, s,
+                                  s, s, alpha, A, s, B, s, beta, C, s, nullptr);
+        } else if (form == "bf16x3") {
             st = sgemm_bf16x3(s, s, s, alpha, A, s, B, s, beta, C, s, nullptr);
         } else if (form == "rocblas") {
             if (rocblas_sgemm(handle, rocblas_operation_none, rocblas_operation_none, s, s, s,
@@ -78,7 +83,8 @@ int main(int argc, char** argv) {
     std::string fname = flag(argc, argv, "fname", "sgemm.cu");
     if (form == "rocblas") fname = "rocBLAS";
 
-    Runner run(form);
+    const int config = flag_int(argc, argv, "config", -1);
+    Runner run(form, config);
     CacheFlusher flusher;
     const float alpha = 1.0f, beta = 0.0f;
 
