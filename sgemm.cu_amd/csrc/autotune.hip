@@ -89,12 +89,18 @@ int heuristic(int form, int grouped, int m, int n, int k) {
         const long long t128 = (long long)div_up(m, 128) * div_up(n, 128);
         if (k <= 256 && t128 >= 512) return 2;
         if (t128 >= 700) return 9;
+        // From 350 tiles with k >= 1024: config 5's tile with stream-K over
+        // all tiles (config 11; no full round of 768 workgroups exists here):
+        // 2432^3 120 -> 124, 2560^3 121 -> 123, 2816^3 123 -> 128 TFLOP/s,
+        // 8192x1024x8192 132 -> 133.
+        if (k >= 1024 && t128 >= 350) return 11;
         // Below that, 64x64 tiles (config 7: loads two k-tiles ahead,
         // split-K when the grid holds fewer than about four workgroups per
         // CU) unless their tiles load the CUs unevenly: with t tiles per CU
         // on average, the busiest CU runs ceil(t).  Under 90 % of that, and
         // from 180 tiles of 128x128 with k >= 1024, config 5's tile with
-        // split-K (config 10) is faster (square sweep 1024..5120):
+        // split-K (config 10) is faster (square sweep 1024..5120; 2560^3 and
+        // 3328^3 now take config 11):
         //   1792^3 (3.1 64x64 tiles per CU) 96 -> 105, 2304^3 (5.1) 108 ->
         //   119, 2560^3 (6.25) 115 -> 121 TFLOP/s;
         // while evenly loaded sizes keep 64x64: 2688^3 (6.9) 127 vs 112,

@@ -55,6 +55,10 @@ constexpr int kX3WideSplitKtiles = 16;
 // tile) and 3584^3 (fifteen).
 constexpr double kX3StreamKPieceIters = 7.0;
 
+// The same for plain FP32 config 11 (config 5's 128x128x16 tile, three
+// workgroups per CU; 64 KB of partials per piece).
+constexpr double kF32StreamKPieceIters = 4.0;
+
 // Stream-K over the `rem` tiles of a partial round (or all tiles, without a
 // full round) of `iters` k-tiles each, spread over `slots` workgroups:
 // returns the k-tiles per workgroup w that minimises w + kPieceIters x

@@ -34,11 +34,53 @@ void k_sgemm_f32(Problem P, SplitK sk) {
     }
 }
 
+// Stream-K over the tail, as k_sgemm_bf16x3_sk: workgroups [0, dp_tiles)
+// compute the tiles of the full rounds whole; workgroup dp_tiles + g of the
+// rest runs k-tile iterations [g * iters, (g + 1) * iters) of the remaining
+// tiles, tile-major with k-tiles in order; a tile cut at a workgroup boundary
+// is reduced from its pieces in k order by the last of them to finish.
+template <class T, bool VEC>
+__global__ __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(T::WPE)))
+void k_sgemm_f32_sk(Problem P, SplitK sk) {
+    __shared__ __attribute__((aligned(16))) float smem[T::LDS_FLOATS];
+    int tm, tn;
+    const int b = blockIdx.x;
+    if (b < sk.dp_tiles) {
+        tile_coords(xcd_remap(b, sk.dp_tiles), P.tiles_m, P.tiles_n, kGroupN, tm, tn);
+        F32TileRunner<T, VEC>::run(P, tm, tn, smem);
+        return;
+    }
+    const int nwg = (int)gridDim.x - sk.dp_tiles;
+    const long long g = xcd_remap(b - sk.dp_tiles, nwg);
+    const int I = div_up(P.k, T::BK);  // k-tiles per tile
+    const long long L = ((long long)P.tiles_m * P.tiles_n - sk.dp_tiles) * I;
+    const long long w = sk.iters;
+    long long x = g * w;
+    const long long end = min(L, x + w);
+    while (x < end) {
+        const int t = (int)(x / I);
+        const int kt0 = (int)(x - (long long)t * I);
+        const int kt1 = (int)min((long long)I, kt0 + (end - x));
+        const int first = (int)((long long)t * I / w);          // workgroup of k-tile 0
+        const int last = (int)(((long long)t * I + I - 1) / w);  // ... and of the last
+        tile_coords(sk.dp_tiles + t, P.tiles_m, P.tiles_n, kGroupN, tm, tn);
+        F32TileRunner<T, VEC>::run_piece(
+            P, tm, tn, smem, &sk,
+            KPiece{kt0 * T::BK, min(P.k, kt1 * T::BK), (int)g - first, last - first + 1, last,
+                   sk.pieces});
+        x = (long long)t * I + kt1;
+        // The next piece's LDS stores must not overtake another wave's read
+        // of the arrival flag in LDS.
+        __syncthreads();
+    }
+}
+
 // Ids 6-8 run the tiles of ids 0-2 with split-K (splitk = 1).  Id 9 runs
 // the tile of id 5 with a tail split (splitk = 2): whole tiles for every full
 // round of the resident workgroups, split-K for the tiles of the last,
 // partial round.  Id 10 runs the tile of id 5 with split-K over the whole
-// grid, each split at least eight k-tiles deep.
+// grid, each split at least eight k-tiles deep.  Id 11 runs the tile of id
+// 5 with stream-K over the tiles of a partial last round (splitk = 3).
 static const ConfigInfo kF32Configs[] = {
     {128, 128, 32, 2, 2, "f32_128x128x32_w2x2"},
     {64, 64, 32, 2, 2, "f32_64x64x32_w2x2"},
@@ -51,6 +93,7 @@ static const ConfigInfo kF32Configs[] = {
     {128, 64, 32, 2, 2, "f32_128x64x32_w2x2_splitk", 1},
     {128, 128, 16, 2, 2, "f32_128x128x16_w2x2_3wpe_tailsplit", 2},
     {128, 128, 16, 2, 2, "f32_128x128x16_w2x2_3wpe_splitk", 1},
+    {128, 128, 16, 2, 2, "f32_128x128x16_w2x2_3wpe_streamk", 3},
 };
 
 int f32_config_count() { return (int)(sizeof(kF32Configs) / sizeof(kF32Configs[0])); }
@@ -154,6 +197,58 @@ static hipError_t launch_tail_split(const Problem& p, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Stream-K over the tail (launch_x3_streamk's scheme on config 5's tile):
+// the r = tiles % slots tiles after the full rounds (all tiles without a full
+// round) spread, k-tile by k-tile, over up to `slots` workgroups of w k-tiles
+// each (streamk_iters).  Falls back to one launch of whole tiles when there
+// is no partial round, or when the stream is being captured and its
+// workspace would have to grow.
+template <class T>
+static hipError_t launch_streamk(const Problem& p, hipStream_t s,
+                                 double piece_iters = kF32StreamKPieceIters) {
+    Problem q = p;
+    q.tiles_m = div_up(p.m, T::BM);
+    q.tiles_n = div_up(p.n, T::BN);
+    q.tile_begin = 0;
+    const long long tiles = (long long)q.tiles_m * q.tiles_n;
+    if (tiles <= 0) return hipSuccess;
+    if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
+    const long long slots = resident_workgroups<T>();
+    const long long rem = tiles % slots;
+    const size_t split_bytes = (size_t)T::MB * T::NB * T::NT * 16;
+    const long long I = div_up(p.k, T::BK);
+    SplitK sk{nullptr, nullptr, 1, p.k, 0};
+    long long nwg = 0;
+    if (rem > 0 && p.k > 0) {
+        double part = 1.0;
+        const long long w = streamk_iters(rem, slots, I, split_bytes, piece_iters, &part);
+        nwg = w > 0 ? div_up_ll(rem * I, w) : 0;
+        sk.iters = (int)w;
+        sk.pieces = w > 0 ? (int)((I - 1) / w + 2) : 0;
+        if (nwg > 1) {
+            const hipError_t e = splitk_workspace(s, (size_t)nwg * sk.pieces * split_bytes,
+                                                  (int)nwg, &sk.partials, &sk.counters);
+            if (e == hipErrorStreamCaptureUnsupported) nwg = 0;
+            else if (e != hipSuccess) return e;
+        } else {
+            nwg = 0;
+        }
+    }
+    (void)hipGetLastError();
+    const bool vec = vec4_ok(p);
+    if (nwg > 0) {
+        sk.dp_tiles = (int)(tiles - rem);
+        const dim3 grid((unsigned)(tiles - rem + nwg));
+        if (vec) hipLaunchKernelGGL((k_sgemm_f32_sk<T, true>), grid, dim3(T::NT), 0, s, q, sk);
+        else hipLaunchKernelGGL((k_sgemm_f32_sk<T, false>), grid, dim3(T::NT), 0, s, q, sk);
+    } else {
+        const dim3 grid((unsigned)tiles);
+        if (vec) hipLaunchKernelGGL((k_sgemm_f32<T, true, false>), grid, dim3(T::NT), 0, s, q, sk);
+        else hipLaunchKernelGGL((k_sgemm_f32<T, false, false>), grid, dim3(T::NT), 0, s, q, sk);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_f32(int id, const Problem& p, hipStream_t s) {
     if (id < 0 || id >= f32_config_count()) return hipErrorInvalidValue;
     switch (id) {
@@ -162,6 +257,7 @@ hipError_t launch_f32(int id, const Problem& p, hipStream_t s) {
         case 8: return launch_tile<F32_128x64, true>(p, s);
         case 9: return launch_tail_split<F32_128x128_k16>(p, s);
         case 10: return launch_tile<F32_128x128_k16, true, 8>(p, s);
+        case 11: return launch_streamk<F32_128x128_k16>(p, s);
         default: break;
     }
     hipError_t e = hipErrorInvalidValue;

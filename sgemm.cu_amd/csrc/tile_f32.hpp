@@ -133,6 +133,19 @@ struct F32TileRunner {
     // the tile's partials and arrival counter.
     __device__ static void run(const Problem& P, int tm, int tn, float* smem,
                                const SplitK* sk = nullptr, int split = 0, int slot = 0) {
+        if (!sk) {
+            run_piece(P, tm, tn, smem, nullptr, KPiece{0, P.k, 0, 1, 0, 1});
+            return;
+        }
+        const int kbeg = split * sk->kchunk;
+        run_piece(P, tm, tn, smem, sk,
+                  KPiece{kbeg, min(P.k, kbeg + sk->kchunk), split, sk->splits, slot, sk->splits});
+    }
+
+    // As run, for the k range and reduction slot of `pc` (pc.n == 1: the
+    // whole tile's k range, stored straight to C).
+    __device__ static void run_piece(const Problem& P, int tm, int tn, float* smem,
+                                     const SplitK* sk, const KPiece pc) {
         const int t = threadIdx.x;
         const int lane = t & 63, wave = t >> 6;
         const int wm = wave % T::WM, wn = wave / T::WM;
@@ -239,8 +252,8 @@ struct F32TileRunner {
 #pragma unroll
             for (int j = 0; j < T::NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-        const int kbeg = sk ? split * sk->kchunk : 0;
-        const int kend = sk ? min(K, kbeg + sk->kchunk) : K;
+        const int kbeg = pc.kbeg;
+        const int kend = pc.kend;
         const int ktiles = kend > kbeg ? div_up(kend - kbeg, BK) : 0;
         if (ktiles > 0) {
             gload(kbeg, Stage0{});
@@ -358,22 +371,23 @@ struct F32TileRunner {
             __syncthreads();
         }
 
-        if (sk) {
+        if (sk && pc.n > 1) {
             // Dump the accumulators in lane order (coalesced 16-byte stores),
-            // then let the last workgroup of this tile reduce in split order.
-            const int tile = slot;
+            // then let the last workgroup of this tile reduce its pieces in k
+            // order.
+            const int tile = pc.region;
             constexpr int NACC = T::MB * T::NB;
             constexpr int SPLIT_BYTES = NACC * T::NT * 16;
             const __amdgpu_buffer_rsrc_t part =
-                make_rsrc(sk->partials + (size_t)tile * sk->splits * NACC * T::NT * 4,
-                          (long long)sk->splits * SPLIT_BYTES);
+                make_rsrc(sk->partials + (size_t)tile * pc.stride * NACC * T::NT * 4,
+                          (long long)pc.stride * SPLIT_BYTES);
 #pragma unroll
             for (int i = 0; i < T::MB; ++i)
 #pragma unroll
                 for (int j = 0; j < T::NB; ++j)
-                    store_agent(part, split * SPLIT_BYTES + ((i * T::NB + j) * T::NT + t) * 16,
+                    store_agent(part, pc.q * SPLIT_BYTES + ((i * T::NB + j) * T::NT + t) * 16,
                                 acc[i][j]);
-            if (!splitk_arrive(*sk, tile, reinterpret_cast<int*>(smem), sk->splits)) return;
+            if (!splitk_arrive(*sk, tile, reinterpret_cast<int*>(smem), pc.n)) return;
             // Sum the partials in split order, this workgroup's own re-read
             // like the others (bit-identical to its registers), so the sum
             // reuses the accumulators' registers.  One split's loads are all
@@ -383,7 +397,7 @@ struct F32TileRunner {
 #pragma unroll
                 for (int j = 0; j < T::NB; ++j)
                     acc[i][j] = load_agent(part, ((i * T::NB + j) * T::NT + t) * 16);
-            for (int q = 1; q < sk->splits; ++q) {
+            for (int q = 1; q < pc.n; ++q) {
 #pragma unroll
                 for (int i = 0; i < T::MB; ++i)
 #pragma unroll

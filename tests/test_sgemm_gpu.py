@@ -24,7 +24,7 @@ from gpu_util import (Case, assert_ok, check, compact, download, log, padding_un
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gemm_cases.json")
-N_F32_CONFIGS = 11
+N_F32_CONFIGS = 12
 
 
 @pytest.fixture(scope="module")
@@ -136,7 +136,8 @@ def test_beta_zero_does_not_read_nan_c():
 
 @pytest.mark.parametrize("config,m,n,k", [(6, 190, 77, 2000), (7, 190, 77, 2000),
                                            (8, 190, 77, 2000), (10, 190, 77, 2000),
-                                           (9, 3584, 3584, 1024)])
+                                           (9, 3584, 3584, 1024), (11, 2000, 1900, 1536),
+                                           (11, 3584, 3584, 1024)])
 def test_beta_zero_nan_c_with_split_k(config, m, n, k):
     """beta == 0 never reads C on the split paths either: the last workgroup
     of a split tile runs the same epilogue (a NaN read would reach every
@@ -450,6 +451,83 @@ def test_tail_split_graph_replays_and_capture_fallback():
         g2.replay()
         torch.cuda.synchronize()
         assert np.array_equal(download(dC), unsplit), rep
+
+
+# ------------------------------------------------------------ stream-K
+STREAMK_CONFIG = 11  # config 5's tile; stream-K over a partial round's tiles
+
+
+@pytest.mark.parametrize("m,n,k,ld_pad", [
+    (1000, 1100, 2000, 3),   # 72 tiles of 128x128, no full round of 768 workgroups
+    (2816, 2816, 1536, 0),   # 484 tiles
+    (3584, 3584, 1024, 1),   # 784 tiles: 16 past one round
+])
+def test_streamk_deterministic_and_in_tolerance(m, n, k, ld_pad):
+    """In FP32 tolerance against FP64 (every entry below 2^21 of them,
+    sampled above), bit-identical on repeat with another problem of the same
+    shape in between (the pieces' partial slots and counters are reused)."""
+    c = Case(m, n, k, lda=m + ld_pad, ldb=k + ld_pad, ldc=m + ld_pad, seed=3 * m + n + k,
+             alpha=1.25, beta=0.5)
+    first = run_form(c, sa.FORM_F32, STREAMK_CONFIG)
+    if m * n <= (1 << 21):
+        assert_ok(check(c, first, orc.fp32_tol_mult(k)), f"stream-K {(m, n, k)}")
+    else:
+        _sampled_check(c, first, n_samples=4096, seed=k)
+    assert padding_untouched(first, c.C0, m, n, c.ldc)
+    other = Case(m, n, k, lda=m + ld_pad, ldb=k + ld_pad, ldc=m + ld_pad, seed=m + n + k + 13,
+                 alpha=1.0, beta=0.0)
+    out = run_form(other, sa.FORM_F32, STREAMK_CONFIG)
+    _sampled_check(other, out, n_samples=2048, seed=k + 5)
+    again = run_form(c, sa.FORM_F32, STREAMK_CONFIG)
+    assert np.array_equal(first, again), (m, n, k)
+
+
+def test_streamk_integer_kat_equals_whole_tiles():
+    """Integer inputs: every partial sum is an exact integer, so stream-K
+    equals config 5 (whole tiles) bit for bit."""
+    m, n, k = 2816, 2432, 1536
+    c = Case(m, n, k, kind="int", seed=29, alpha=1.5, beta=0.5)
+    assert np.array_equal(run_form(c, sa.FORM_F32, 5), run_form(c, sa.FORM_F32, STREAMK_CONFIG))
+
+
+def test_streamk_graph_replays_and_capture_fallback():
+    """Captured after a warm-up call on the stream, stream-K replays
+    bit-identically to its eager result; captured on a fresh stream (no
+    workspace yet) it runs config 5's kernel on whole tiles, bit for bit."""
+    import torch
+    m, n, k = 2816, 2560, 1024
+    c = Case(m, n, k, seed=81, alpha=1.0, beta=0.0)
+    dA, dB, dC = c.device()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        sa.run_config(0, STREAMK_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, s)
+        s.synchronize()
+        eager = download(dC).copy()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            sa.run_config(0, STREAMK_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m,
+                          torch.cuda.current_stream())
+    for rep in range(3):
+        dC.fill_(5.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(download(dC), eager), rep
+    sa.run_config(0, 5, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m)
+    torch.cuda.synchronize()
+    whole = download(dC).copy()
+    assert not np.array_equal(whole, eager)  # the tiles were cut
+    fresh = torch.cuda.Stream()
+    g2 = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g2, stream=fresh):
+        sa.run_config(0, STREAMK_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m,
+                      torch.cuda.current_stream())
+    for rep in range(2):
+        dC.fill_(-2.0)
+        torch.cuda.synchronize()
+        g2.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(download(dC), whole), rep
 
 
 @pytest.mark.parametrize("form", [sa.FORM_F32, sa.FORM_BF16X3])
