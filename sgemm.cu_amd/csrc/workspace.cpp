@@ -97,14 +97,25 @@ long long streamk_iters(long long rem, long long slots, long long iters, size_t 
     if (rem <= 0 || slots <= 0 || iters <= 0) return 0;
     const long long L = rem * iters;
     const long long w0 = std::max(1LL, (L + slots - 1) / slots);
+    // For a piece bound p (2..16) the cheapest range length is the shortest
+    // w >= w0 with at most p pieces per tile, (iters - 1) / w + 2 <= p, i.e.
+    // w >= (iters - 1) / (p - 1) + 1, and with the partials of ceil(L / w)
+    // workgroups x p pieces within the cap; w0 itself is the other candidate.
+    constexpr double kCap = 512.0 * 1024 * 1024;
     long long best = 0;
     double best_t = 0;
-    for (long long w = w0; w <= std::max(w0, iters); ++w) {
+    for (int p = 1; p <= 16; ++p) {
+        long long w = w0;
+        if (p > 1) {
+            const long long nmax = (long long)(kCap / ((double)p * split_bytes));
+            if (nmax < 1) continue;
+            w = std::max({w0, (iters - 1) / (p - 1) + 1, (L + nmax - 1) / nmax});
+        }
         const long long pieces = (iters - 1) / w + 2;
         if (pieces > 16) continue;
-        if ((double)((L + w - 1) / w) * pieces * split_bytes > 512.0 * 1024 * 1024) continue;
+        if ((double)((L + w - 1) / w) * pieces * split_bytes > kCap) continue;
         const double t = (double)w + piece_iters * pieces;
-        if (best == 0 || t < best_t) {
+        if (best == 0 || t < best_t || (t == best_t && w < best)) {
             best = w;
             best_t = t;
         }
