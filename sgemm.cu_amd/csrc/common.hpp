@@ -102,9 +102,6 @@ __device__ inline bool splitk_arrive(const SplitK& sk, int tile, int* flag_lds, 
 static_assert(sizeof(Problem) % 8 == 0, "descriptor packing");
 
 __host__ __device__ constexpr int div_up(int a, int b) { return (a + b - 1) / b; }
-__host__ __device__ constexpr long long div_up_ll(long long a, long long b) {
-    return (a + b - 1) / b;
-}
 
 // Elements a column-major (rows x cols, ld) operand spans.
 __host__ __device__ inline long long extent(int rows, int cols, int ld) {
@@ -166,6 +163,38 @@ __device__ inline void tile_coords(int id, int tiles_m, int tiles_n, int group_n
     const int local = id - band * per_band;
     tm = local / width;
     tn = first_n + local % width;
+}
+
+// Stream-K part of a launch whose first sk.dp_tiles workgroups run whole
+// tiles: calls piece(tile, KPiece) for each piece of stream-K workgroup
+// `b - sk.dp_tiles`, in order.  That workgroup runs k-tile iterations [g *
+// iters, (g + 1) * iters) of the tiles after dp_tiles, tile-major with bk-deep
+// k-tiles in order (g = the XCD-remapped index); a tile's piece q of n is
+// reduced through region `last` (the workgroup holding its last k-tile).
+// Every thread of the workgroup calls this; `piece` must leave the workgroup
+// converged, and the barrier between pieces keeps the next piece's LDS
+// stores behind every wave's read of the previous one's arrival flag.
+template <class F>
+__device__ inline void streamk_pieces(const Problem& P, const SplitK& sk, int b, int bk,
+                                      F&& piece) {
+    const int nwg = (int)gridDim.x - sk.dp_tiles;
+    const long long g = xcd_remap(b - sk.dp_tiles, nwg);
+    const int I = div_up(P.k, bk);  // k-tiles per tile
+    const long long L = ((long long)P.tiles_m * P.tiles_n - sk.dp_tiles) * I;
+    const long long w = sk.iters;
+    long long x = g * w;
+    const long long end = min(L, x + w);
+    while (x < end) {
+        const int t = (int)(x / I);
+        const int kt0 = (int)(x - (long long)t * I);
+        const int kt1 = (int)min((long long)I, kt0 + (end - x));
+        const int first = (int)((long long)t * I / w);          // workgroup of k-tile 0
+        const int last = (int)(((long long)t * I + I - 1) / w);  // ... and of the last
+        piece(sk.dp_tiles + t, KPiece{kt0 * bk, min(P.k, kt1 * bk), (int)g - first,
+                                      last - first + 1, last, sk.pieces});
+        x = (long long)t * I + kt1;
+        __syncthreads();
+    }
 }
 
 }  // namespace sgemm_amd

@@ -68,6 +68,15 @@ constexpr double kF32StreamKPieceIters = 4.0;
 // 0 when no w qualifies.
 long long streamk_iters(long long rem, long long slots, long long iters, size_t split_bytes,
                         double piece_iters, double* part);
+
+// Plans stream-K over the tail of a grid of `tiles` tiles on `slots`
+// resident workgroups: sets sk->iters, sk->pieces and the workspace (partials
+// and counters for *nwg stream-K workgroups) and returns in *nwg the number of
+// stream-K workgroups, 0 when the launch should run whole tiles only (no
+// partial round, one stream-K workgroup would suffice, or the stream is being
+// captured and its workspace would have to grow).
+hipError_t streamk_plan(hipStream_t stream, long long tiles, long long slots, int k, int bk,
+                        size_t split_bytes, double piece_iters, SplitK* sk, long long* nwg);
 // Plain FP32 rate of one CU running config 5's tile three at a time (141
 // TFLOP/s over 256 CUs at 8192^3).
 constexpr double kF32FlopsPerCuUs = 0.55e6;

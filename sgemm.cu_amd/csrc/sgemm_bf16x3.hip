@@ -49,29 +49,10 @@ void k_sgemm_bf16x3_sk(Problem P, SplitK sk) {
         X3TileRunner<T, VEC>::run(P, tm, tn, smem);
         return;
     }
-    const int nwg = (int)gridDim.x - sk.dp_tiles;
-    const long long g = xcd_remap(b - sk.dp_tiles, nwg);
-    const int I = div_up(P.k, T::BK);  // k-tiles per tile
-    const long long L = ((long long)P.tiles_m * P.tiles_n - sk.dp_tiles) * I;
-    const long long w = sk.iters;
-    long long x = g * w;
-    const long long end = min(L, x + w);
-    while (x < end) {
-        const int t = (int)(x / I);
-        const int kt0 = (int)(x - (long long)t * I);
-        const int kt1 = (int)min((long long)I, kt0 + (end - x));
-        const int first = (int)((long long)t * I / w);          // workgroup of k-tile 0
-        const int last = (int)(((long long)t * I + I - 1) / w);  // ... and of the last
-        tile_coords(sk.dp_tiles + t, P.tiles_m, P.tiles_n, kX3GroupN, tm, tn);
-        X3TileRunner<T, VEC>::run_piece(
-            P, tm, tn, smem, &sk,
-            KPiece{kt0 * T::BK, min(P.k, kt1 * T::BK), (int)g - first, last - first + 1, last,
-                   sk.pieces});
-        x = (long long)t * I + kt1;
-        // The next piece's LDS stores must not overtake another wave's read
-        // of the arrival flag in LDS.
-        __syncthreads();
-    }
+    streamk_pieces(P, sk, b, T::BK, [&](int tile, const KPiece& pc) {
+        tile_coords(tile, P.tiles_m, P.tiles_n, kX3GroupN, tm, tn);
+        X3TileRunner<T, VEC>::run_piece(P, tm, tn, smem, &sk, pc);
+    });
 }
 
 // Ids 6-7 run the tiles of ids 0-1 with split-K (splitk = 1); id 8 is a
@@ -216,25 +197,11 @@ static hipError_t launch_x3_streamk(const Problem& p, hipStream_t s) {
     const long long rem = tiles % slots;
     constexpr size_t acc_bytes = T::MF == 16 ? 16 : 64;
     const size_t split_bytes = (size_t)T::MB * T::NB * acc_bytes * T::NT;
-    const long long I = div_up(p.k, T::BK);
     SplitK sk{nullptr, nullptr, 1, p.k, 0};
     long long nwg = 0;
-    if (rem > 0 && p.k > 0) {
-        const long long L = rem * I;
-        double part = 1.0;
-        const long long w = streamk_iters(rem, slots, I, split_bytes, kX3StreamKPieceIters, &part);
-        nwg = w > 0 ? div_up_ll(L, w) : 0;
-        sk.iters = (int)w;
-        sk.pieces = w > 0 ? (int)((I - 1) / w + 2) : 0;
-        if (nwg > 1) {
-            const hipError_t e = splitk_workspace(s, (size_t)nwg * sk.pieces * split_bytes,
-                                                  (int)nwg, &sk.partials, &sk.counters);
-            if (e == hipErrorStreamCaptureUnsupported) nwg = 0;
-            else if (e != hipSuccess) return e;
-        } else {
-            nwg = 0;
-        }
-    }
+    const hipError_t e =
+        streamk_plan(s, tiles, slots, p.k, T::BK, split_bytes, kX3StreamKPieceIters, &sk, &nwg);
+    if (e != hipSuccess) return e;
     (void)hipGetLastError();
     const bool vec = vec4_ok(p);
     if (nwg > 0) {

@@ -124,6 +124,27 @@ long long streamk_iters(long long rem, long long slots, long long iters, size_t 
     return best;
 }
 
+hipError_t streamk_plan(hipStream_t stream, long long tiles, long long slots, int k, int bk,
+                        size_t split_bytes, double piece_iters, SplitK* sk, long long* nwg) {
+    *nwg = 0;
+    const long long rem = slots > 0 ? tiles % slots : 0;
+    if (rem <= 0 || k <= 0) return hipSuccess;
+    const long long iters = div_up(k, bk);
+    double part = 1.0;
+    const long long w = streamk_iters(rem, slots, iters, split_bytes, piece_iters, &part);
+    if (w <= 0) return hipSuccess;
+    const long long n = (rem * iters + w - 1) / w;
+    if (n <= 1) return hipSuccess;
+    sk->iters = (int)w;
+    sk->pieces = (int)((iters - 1) / w + 2);
+    const hipError_t e = splitk_workspace(stream, (size_t)n * sk->pieces * split_bytes, (int)n,
+                                          &sk->partials, &sk->counters);
+    if (e == hipErrorStreamCaptureUnsupported) return hipSuccess;
+    if (e != hipSuccess) return e;
+    *nwg = n;
+    return hipSuccess;
+}
+
 hipError_t splitk_plan(hipStream_t stream, long long tiles, int k, int bk, size_t split_bytes,
                        SplitK* sk, bool f32, int min_ktiles) {
     sk->partials = nullptr;
