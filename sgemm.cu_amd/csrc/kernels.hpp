@@ -55,6 +55,13 @@ constexpr int kX3WideSplitKtiles = 16;
 // tile) and 3584^3 (fifteen).
 constexpr double kX3StreamKPieceIters = 7.0;
 
+// Configs that balance a partial last round of tiles over the whole GPU --
+// the tail split (9) and stream-K (11) of either form -- and the config that
+// runs the same tile whole (FP32 5, BF16x3 2); -1 for any other config.
+constexpr int whole_tile_twin(int form, int config) {
+    return (config == 9 || config == 11) ? (form == kFormBf16x3 ? 2 : 5) : -1;
+}
+
 // The same for plain FP32 config 11 (config 5's 128x128x16 tile, three
 // workgroups per CU; 64 KB of partials per piece).
 constexpr double kF32StreamKPieceIters = 4.0;
