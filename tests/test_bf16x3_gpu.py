@@ -272,7 +272,7 @@ X3_STREAMK_CONFIG = 11  # config 2's 256x256 tile; stream-K over a partial round
 
 @pytest.mark.parametrize("m,n,k,ld_pad", [
     (1000, 1100, 2000, 3),   # 20 tiles, no full round: every tile cut into many pieces
-    (3000, 3100, 1500, 0),   # 156 tiles
+    (3000, 3100, 1501, 0),   # 156 tiles; ragged last k-tile
     (4352, 4096, 1024, 0),   # 272 tiles: 16 past one 256-CU round
     (5120, 5120, 640, 1),    # 400 tiles: 144 past one round
 ])

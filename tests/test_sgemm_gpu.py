@@ -458,7 +458,7 @@ STREAMK_CONFIG = 11  # config 5's tile; stream-K over a partial round's tiles
 
 
 @pytest.mark.parametrize("m,n,k,ld_pad", [
-    (1000, 1100, 2000, 3),   # 72 tiles of 128x128, no full round of 768 workgroups
+    (1000, 1100, 2001, 3),   # 72 tiles of 128x128, no full round; ragged last k-tile
     (2816, 2816, 1536, 0),   # 484 tiles
     (3584, 3584, 1024, 1),   # 784 tiles: 16 past one round
 ])
