@@ -109,6 +109,12 @@ int heuristic(int form, int grouped, int m, int n, int k) {
             const double t = (double)div_up(m, 64) * div_up(n, 64) / device_cu_count();
             if (t / std::ceil(t) < 0.9) return 10;
         }
+        // One 64x64 tile per CU (90-100 % of the CUs) and k <= 1536: the
+        // k-group tile (config 12) over split-K: 1024^3 86 -> 92, 1000^3 79
+        // -> 84 TFLOP/s (at k = 4096 split-K wins, 114 vs 105).
+        const long long t64 = (long long)div_up(m, 64) * div_up(n, 64);
+        const long long cus = device_cu_count();
+        if (k <= 1536 && t64 <= cus && 10 * t64 >= 9 * cus) return 12;
         return 7;
     }
     if (grouped) return (m >= 96 && n >= 96) ? 8 : 1;

@@ -16,6 +16,7 @@ constexpr int kGroupN = 8;
 template <class T, bool VEC, bool SPLIT>
 __global__ __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(T::WPE)))
 void k_sgemm_f32(Problem P, SplitK sk) {
+    static_assert(!SPLIT || T::KG == 1, "k-group tiles run whole tiles only");
     __shared__ __attribute__((aligned(16))) float smem[T::LDS_FLOATS];
     int tm, tn;
     const int b = blockIdx.x;
@@ -61,7 +62,8 @@ void k_sgemm_f32_sk(Problem P, SplitK sk) {
 // round of the resident workgroups, split-K for the tiles of the last,
 // partial round.  Id 10 runs the tile of id 5 with split-K over the whole
 // grid, each split at least eight k-tiles deep.  Id 11 runs the tile of id
-// 5 with stream-K over the tiles of a partial last round (splitk = 3).
+// 5 with stream-K over the tiles of a partial last round (splitk = 3).  Id
+// 12 is a 64x64x64 tile with two k groups of four waves (F32_64x64_kg2).
 static const ConfigInfo kF32Configs[] = {
     {128, 128, 32, 2, 2, "f32_128x128x32_w2x2"},
     {64, 64, 32, 2, 2, "f32_64x64x32_w2x2"},
@@ -75,6 +77,7 @@ static const ConfigInfo kF32Configs[] = {
     {128, 128, 16, 2, 2, "f32_128x128x16_w2x2_3wpe_tailsplit", 2},
     {128, 128, 16, 2, 2, "f32_128x128x16_w2x2_3wpe_splitk", 1},
     {128, 128, 16, 2, 2, "f32_128x128x16_w2x2_3wpe_streamk", 3},
+    {64, 64, 64, 2, 2, "f32_64x64x64_w2x2_kg2"},
 };
 
 int f32_config_count() { return (int)(sizeof(kF32Configs) / sizeof(kF32Configs[0])); }
@@ -225,6 +228,7 @@ hipError_t launch_f32(int id, const Problem& p, hipStream_t s) {
         case 9: return launch_tail_split<F32_128x128_k16>(p, s);
         case 10: return launch_tile<F32_128x128_k16, true, 8>(p, s);
         case 11: return launch_streamk<F32_128x128_k16>(p, s);
+        case 12: return launch_tile<F32_64x64_kg2, false>(p, s);
         default: break;
     }
     hipError_t e = hipErrorInvalidValue;

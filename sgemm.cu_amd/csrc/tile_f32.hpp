@@ -43,15 +43,21 @@ namespace sgemm_amd {
 // first k-step.  PF > 1 keeps PF register stages, so the loads of t+PF are
 // issued PF-1 k-tiles before they are stored: for grids with one workgroup
 // per CU, where no other workgroup covers the memory latency.
+// KG_: k groups.  KG x WM x WN waves; group g multiplies the 16-deep k-steps
+// g, g+KG, ... of every k-tile into its own accumulators, and the groups'
+// sums are added in group order through LDS before the epilogue.  Two groups
+// put two waves of the tile on each SIMD, so one covers the other's LDS and
+// barrier waits.  Whole tiles only (no split-K).
 template <int BM_, int BN_, int BK_, int WM_, int WN_, bool AT_ = false, int WPE_ = 1,
-          int SCHED_ = 0, int PF_ = 1>
+          int SCHED_ = 0, int PF_ = 1, int KG_ = 1>
 struct F32Tile {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
     static constexpr bool AT = AT_;
     static constexpr int WPE = WPE_;
     static constexpr int SCHED = SCHED_;
     static constexpr int PF = PF_;
-    static constexpr int NT = 64 * WM * WN;
+    static constexpr int KG = KG_;
+    static constexpr int NT = 64 * WM * WN * KG;
     static constexpr int TM = BM / WM, TN = BN / WN;  // wave tile
     static constexpr int MB = TM / 16, NB = TN / 16;  // 16x16 MFMA blocks per wave
     // A layout: As[k][m] with row stride BM+4, or As[m][k] with row stride BK+4.
@@ -72,6 +78,9 @@ struct F32Tile {
                   "transposed A needs whole 4x4 sub-blocks per thread");
     static_assert(LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
     static_assert(PF >= 1 && PF <= 4, "prefetch depth");
+    static_assert(KG >= 1 && BK % (16 * KG) == 0, "k groups split the k-tile's k-steps");
+    static_assert(KG == 1 || (KG - 1) * (TM / 16) * (TN / 16) * 64 * WM * WN * 4 <= LDS_FLOATS,
+                  "k-group sums fit in LDS");
 };
 
 // Rows r..r+3 of one column, zero outside [0, valid).  VEC: one 16-byte load
@@ -148,7 +157,10 @@ struct F32TileRunner {
                                      const SplitK* sk, const KPiece pc) {
         const int t = threadIdx.x;
         const int lane = t & 63, wave = t >> 6;
-        const int wm = wave % T::WM, wn = wave / T::WM;
+        constexpr int GW = T::WM * T::WN;  // waves per k group
+        const int kg = T::KG == 1 ? 0 : wave / GW;
+        const int wv = T::KG == 1 ? wave : wave % GW;
+        const int wm = wv % T::WM, wn = wv / T::WM;
         const int l16 = lane & 15, g = lane >> 4;
         const int m0 = tm * BM, n0 = tn * BN;
         const int M = P.m, N = P.n, K = P.k;
@@ -291,7 +303,8 @@ struct F32TileRunner {
             if (T::PF == 1 ? more : kt + T::PF < ktiles) gload(kbeg + (kt + T::PF) * BK, p_tag);
             if constexpr (NEXT_FULL && T::SCHED == 0) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int kk = 0; kk < BK; kk += 16) {
+            for (int kq = 0; kq < BK / 16 / T::KG; ++kq) {
+                const int kk = T::KG == 1 ? 16 * kq : 16 * (kq * T::KG + kg);
                 // Lane group g supplies k = kk + 4g + s at MFMA step s, for A
                 // and B alike, so B's four steps are one 16-byte LDS read.
                 f32x4 a[T::MB];
@@ -322,7 +335,7 @@ struct F32TileRunner {
                 // Stage the next k-tile behind the first k-step's MFMAs, so
                 // the wait on its loads, the masking and the LDS stores
                 // overlap matrix-core work instead of following all of it.
-                if (kk == 0) {
+                if (kq == 0) {
                     if constexpr (NEXT_FULL) {
                         if constexpr (T::SCHED == 0) __builtin_amdgcn_sched_barrier(0);
                         sstore_impl(next_stage, knext, std::false_type{}, NextStage{});
@@ -371,7 +384,33 @@ struct F32TileRunner {
             __syncthreads();
         }
 
-        if (sk && pc.n > 1) {
+        if constexpr (T::KG > 1) {
+            // Groups 1.. hand their sums to group 0 through LDS (free after
+            // the k loop's last barrier); group 0 adds them in group order.
+            constexpr int GT = 64 * GW;
+            f32x4* red = reinterpret_cast<f32x4*>(smem);
+            const int tg = t % GT;
+            if (kg > 0) {
+#pragma unroll
+                for (int i = 0; i < T::MB; ++i)
+#pragma unroll
+                    for (int j = 0; j < T::NB; ++j)
+                        red[(((kg - 1) * T::MB + i) * T::NB + j) * GT + tg] = acc[i][j];
+            }
+            __syncthreads();
+            if (kg == 0) {
+                for (int q = 1; q < T::KG; ++q)
+#pragma unroll
+                    for (int i = 0; i < T::MB; ++i)
+#pragma unroll
+                        for (int j = 0; j < T::NB; ++j)
+                            acc[i][j] += red[(((q - 1) * T::MB + i) * T::NB + j) * GT + tg];
+            }
+            __syncthreads();     // LDS is free again for a caller's next tile
+            if (kg > 0) return;  // the epilogue below has no barrier
+        }
+
+        if (T::KG == 1 && sk && pc.n > 1) {
             // Dump the accumulators in lane order (coalesced 16-byte stores),
             // then let the last workgroup of this tile reduce its pieces in k
             // order.
@@ -450,6 +489,9 @@ using F32_128x64 = F32Tile<128, 64, 32, 2, 2>;
 using F32_256x128 = F32Tile<256, 128, 16, 2, 2>;
 using F32_128x128_at = F32Tile<128, 128, 32, 2, 2, true, 2>;
 using F32_128x128_k16 = F32Tile<128, 128, 16, 2, 2, false, 3, 1>;
+// 64x64x64 with two k groups of four waves, loads two k-tiles ahead: for
+// grids of one 64x64 tile per CU (1024^3: 86 -> 92 TFLOP/s).
+using F32_64x64_kg2 = F32Tile<64, 64, 64, 2, 2, false, 1, 0, 2, 2>;
 
 template <class T>
 struct TypeTag {
