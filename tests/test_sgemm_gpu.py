@@ -530,6 +530,32 @@ def test_streamk_graph_replays_and_capture_fallback():
         assert np.array_equal(download(dC), whole), rep
 
 
+def test_streamk_concurrent_streams_match_sequential():
+    """Each stream has its own stream-K workspace: two stream-K launches of
+    different problems running at the same time on two streams give the bits
+    each gives alone."""
+    import torch
+    shapes = [(2816, 2560, 1536), (2432, 2600, 1024)]
+    cases = [Case(m, n, k, seed=95 + i, alpha=1.0, beta=0.0) for i, (m, n, k) in enumerate(shapes)]
+    devs = [c.device() for c in cases]
+    alone = []
+    for c, (dA, dB, dC) in zip(cases, devs):
+        sa.run_config(0, STREAMK_CONFIG, c.m, c.n, c.k, 1.0, dA, c.m, dB, c.k, 0.0, dC, c.m)
+        torch.cuda.synchronize()
+        alone.append(download(dC).copy())
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for rep in range(3):
+        for (_, _, dC) in devs:
+            dC.fill_(3.0)
+        torch.cuda.synchronize()
+        for c, (dA, dB, dC), st in zip(cases, devs, streams):
+            sa.run_config(0, STREAMK_CONFIG, c.m, c.n, c.k, 1.0, dA, c.m, dB, c.k, 0.0, dC,
+                          c.m, st)
+        torch.cuda.synchronize()
+        for i, (_, _, dC) in enumerate(devs):
+            assert np.array_equal(download(dC), alone[i]), (rep, i)
+
+
 @pytest.mark.parametrize("form", [sa.FORM_F32, sa.FORM_BF16X3])
 def test_result_larger_than_4gib(form):
     """C of 65539 x 16389 floats (4.3 GB): element offsets pass 2^32 bytes, so
