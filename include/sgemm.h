@@ -220,8 +220,11 @@ typedef struct {
     double mock_launch_us;
     double chunk_gflop;      /* device-operand requests above this many GFLOP (2mnk/1e9) are
                                 cut into column panels of about this size, each one launch
-                                interleaved with the small-request batches; 0 = 100 GFLOP,
-                                < 0 = never cut (every request goes into grouped batches) */
+                                interleaved with the small-request batches (the config the
+                                autotune table picks for the panel's shape, except that a
+                                tail-split or stream-K pick runs the same tile whole, since
+                                other slots share the GPU); 0 = 100 GFLOP, < 0 = never cut
+                                (every request goes into grouped batches) */
 } sgemm_server_config;
 
 /* One request.  operands == SGEMM_OPERANDS_DEVICE: A, B, C are device memory
