@@ -244,10 +244,19 @@ float orc_bf16_to_f32(uint16_t h) {
     return f;
 }
 
-/* hi = bf16(x), lo = bf16(x - float(hi)).  x - hi is exact in FP32. */
+/* hi = bf16(clamp(x, -BF16_MAX, BF16_MAX)), lo = bf16(x - float(hi)).
+ * BF16_MAX = 0x7f7f (3.38953139e38) is the largest finite bf16; the clamp is
+ * the identity for |x| <= BF16_MAX, so there hi = bf16(x).  Beyond it hi
+ * stays finite: a finite x keeps an exact residual x - hi (Sterbenz), and
+ * x = +-Inf puts the infinity into lo.  For NaN x only lo = NaN is specified
+ * (the GPU's clamp of a NaN is not pinned); here hi is NaN too. */
 void orc_bf16x3_split(const float* x, size_t n, uint16_t* hi, uint16_t* lo) {
+    const float bmax = orc_bf16_to_f32(0x7f7f);
     for (size_t i = 0; i < n; ++i) {
-        uint16_t h = orc_bf16_rne(x[i]);
+        float c = x[i];
+        if (c > bmax) c = bmax;
+        if (c < -bmax) c = -bmax;
+        uint16_t h = orc_bf16_rne(c);
         float r = x[i] - orc_bf16_to_f32(h);
         hi[i] = h;
         lo[i] = orc_bf16_rne(r);

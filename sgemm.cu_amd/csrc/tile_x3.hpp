@@ -8,11 +8,16 @@
 //
 // Each FP32 input x is split as it is staged into LDS:
 //     hi = bf16_rne(x),  lo = bf16_rne(x - hi)      (x - hi is exact in FP32)
-// and A*B is accumulated in FP32 as  hi*hi + (hi*lo + lo*hi)  on
-// v_mfma_f32_16x16x32_bf16 (BF16 products are exact in FP32).  The small cross
-// terms go to their own accumulators, added to the main ones in the epilogue,
-// so their rounding stays 2^-8 below the main sum's.  Dropped: lo*lo and the
-// residual x - hi - lo, at most ~3*2^-16 |a||b| per product.
+// (hi from x clamped to the largest finite bf16, see split4) and A*B is
+// accumulated in FP32 as  hi*hi + hi*lo + lo*hi  on the BF16 MFMAs (BF16
+// products are exact in FP32).  DUAL configs keep the small cross terms in
+// accumulators of their own, added to the main ones in the epilogue; the
+// others accumulate all three products in one set.  Dropped: lo*lo and the
+// residual x - hi - lo, at most ~3*2^-16 |a||b| per product.  That residual
+// (~2^-17 |x| for a two-term split) sets the error: on wide-range inputs
+// the mean componentwise error is about 3x the plain FP32 kernel's whatever
+// the accumulation order (a separate cross-term set or a fourth lo*lo
+// product changes it by under 20 %).
 //
 // Global -> LDS: 16-byte loads of FP32 into registers for k-tile t+1 during
 // the MFMAs of tile t; the split happens in registers; hi and lo land in LDS
@@ -91,11 +96,25 @@ __device__ inline f32x4 x3_load4(const float* p, int valid) {
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 
+// Largest finite bf16 (0x7f7f), exactly representable in FP32.
+constexpr float kBf16Max = 3.38953139e38f;
+
+// hi is rounded from x clamped to +-kBf16Max (one v_med3_f32, the identity
+// for every |x| <= kBf16Max), so it is always finite:
+//  * a finite |x| above kBf16Max, which would round to Inf, keeps hi =
+//    +-kBf16Max and lo = bf16(x - hi), and x - hi is still exact (Sterbenz);
+//  * x = +-Inf gives lo = +-Inf, so lo*b_hi carries the infinity with the
+//    sign of the FP32 product, and Inf * 0 turns into NaN only where b
+//    itself is 0, as in FP32.  (With hi = Inf and lo = 0 instead, hi*b_lo
+//    would be Inf * 0 = NaN for every bf16-exact b.);
+//  * NaN gives lo = NaN.
 __device__ inline void split4(f32x4 x, bf16x4_t& hi, bf16x4_t& lo) {
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
         const f32x2 v = {x[2 * p], x[2 * p + 1]};
-        const bf16x2_t h = __builtin_convertvector(v, bf16x2_t);
+        const f32x2 c = {__builtin_amdgcn_fmed3f(v[0], -kBf16Max, kBf16Max),
+                         __builtin_amdgcn_fmed3f(v[1], -kBf16Max, kBf16Max)};
+        const bf16x2_t h = __builtin_convertvector(c, bf16x2_t);
         const f32x2 r = v - __builtin_convertvector(h, f32x2);  // exact in FP32
         const bf16x2_t l = __builtin_convertvector(r, bf16x2_t);
         hi[2 * p] = h[0];

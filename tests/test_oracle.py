@@ -224,6 +224,29 @@ def test_bf16x3_split_properties():
     assert np.all(ly == 0)
 
 
+def test_bf16x3_split_non_finite_and_overflow():
+    """hi is rounded from x clamped to +-0x7f7f (the largest finite bf16,
+    (2 - 2^-7) * 2^127), so it never overflows.  Expected bits worked by hand:
+      +Inf -> hi 0x7f7f, lo = bf16(Inf - hi) = +Inf 0x7f80 (and the negatives);
+      FLT_MAX = (2 - 2^-23) 2^127 -> hi 0x7f7f, residual 2^120 - 2^104, which
+        rounds to 2^120: lo 0x7b80 (exponent 247, mantissa 0);
+      0x7f7f8000 (the tie above 0x7f7f, plain RNE goes to Inf) -> hi 0x7f7f,
+        residual exactly 2^119: lo 0x7b00;
+      NaN -> lo is a NaN."""
+    def bits(x):
+        return np.array([struct.unpack("<f", struct.pack("<I", b))[0] for b in x], np.float32)
+    x = bits([0x7F800000, 0xFF800000, 0x7F7FFFFF, 0xFF7FFFFF, 0x7F7F8000, 0x7F7F0000])
+    hi, lo = orc.bf16x3_split(x)
+    assert [int(v) for v in hi] == [0x7F7F, 0xFF7F, 0x7F7F, 0xFF7F, 0x7F7F, 0x7F7F]
+    assert [int(v) for v in lo] == [0x7F80, 0xFF80, 0x7B80, 0xFB80, 0x7B00, 0x0000]
+    # hi + lo reproduces every finite clamped value to 2^-16 relative
+    for xv, h, l in zip(x[2:], hi[2:], lo[2:]):
+        s = orc.bf16_to_f32(int(h)) + orc.bf16_to_f32(int(l))
+        assert abs(s - float(xv)) <= 2.0 ** -16 * abs(float(xv))
+    _, ln = orc.bf16x3_split(np.array([np.nan], np.float32))
+    assert (int(ln[0]) & 0x7F80) == 0x7F80 and (int(ln[0]) & 0x7F) != 0
+
+
 # ------------------------------------------------------ metric / tolerance
 def test_gamma_and_tolerance():
     u = 2.0 ** -24
