@@ -39,7 +39,9 @@ enum {
     SGEMM_STATUS_NOT_READY = 6,       /* a polled request is still in flight */
     SGEMM_STATUS_NOT_FOUND = 7,       /* unknown handle or request id */
     SGEMM_STATUS_IO_ERROR = 8,        /* autotune table could not be read or written */
-    SGEMM_STATUS_WRONG_STATE = 9      /* e.g. replay of a plan that was never captured */
+    SGEMM_STATUS_WRONG_STATE = 9,     /* e.g. replay of a plan that was never captured */
+    SGEMM_STATUS_SCRATCH_TOO_SMALL = 10 /* the problem's split plan needs more than a fixed
+                                           scratch array holds; nothing was launched */
 };
 
 /* Arithmetic forms. */
@@ -96,10 +98,13 @@ const char* sgemm_hip_error_string(void);
  * grids that would hold few workgroups per CU) and the tail-split config
  * (for grids whose last round of resident workgroups is partial) use a device
  * workspace owned by the library, one per stream, allocated on the first such
- * call and kept until the process ends; a call on a stream under graph
- * capture never allocates and runs unsplit if the workspace is too small.
- * Results of a split config are deterministic (partials are summed in split
- * order). */
+ * call and kept until sgemm_workspace_release(stream) or the process ends; a
+ * call on a stream under graph capture never allocates and runs unsplit if
+ * the workspace is too small.  Results of a split config are deterministic
+ * (partials are summed in split order).  Launches that share a stream's
+ * workspace must not overlap: a graph that captured such calls on `stream`
+ * uses that stream's workspace whenever it is replayed, so its replays must
+ * be ordered with each other and with eager calls on `stream`. */
 int sgemm_ex(int m, int n, int k, const float* alpha, const float* A, int lda, const float* B,
              int ldb, const float* beta, float* C, int ldc, sgemm_stream_t stream);
 
@@ -114,6 +119,14 @@ int sgemm_ex(int m, int n, int k, const float* alpha, const float* A, int lda, c
 int sgemm_bf16x3(int m, int n, int k, const float* alpha, const float* A, int lda,
                  const float* B, int ldb, const float* beta, float* C, int ldc,
                  sgemm_stream_t stream);
+
+/* Frees the split-K / stream-K workspace the library holds for `stream`
+ * (waiting for the stream first).  Call it before destroying a stream that
+ * ran split configs, so the handle cannot be reused while its workspace is
+ * still in use; graphs that captured split launches on `stream` must not be
+ * replayed afterwards.  A stream without a workspace is a no-op.  No
+ * reference counterpart (the reference holds no device state). */
+int sgemm_workspace_release(sgemm_stream_t stream);
 
 /* Explicit form and kernel config (config ids from sgemm_config_count /
  * sgemm_config_info; splitk configs as described at sgemm_ex).  Used by the
@@ -225,12 +238,18 @@ typedef struct {
                                 tail-split or stream-K pick runs the same tile whole, since
                                 other slots share the GPU); 0 = 100 GFLOP, < 0 = never cut
                                 (every request goes into grouped batches) */
+    int mock_fail_launch;    /* SGEMM_EXECUTOR_MOCK only: the launch with this 1-based index
+                                (grouped batch or panel) fails, to test failure handling;
+                                0 = none */
 } sgemm_server_config;
 
 /* One request.  operands == SGEMM_OPERANDS_DEVICE: A, B, C are device memory
- * the caller keeps alive until the request completes.  SGEMM_OPERANDS_HOST:
- * host memory; the server copies A, B (and C if beta != 0) into its pool on
- * admission and C back on completion. */
+ * the caller keeps alive until the request has finished (poll/wait return a
+ * status other than SGEMM_STATUS_NOT_READY, success or error: a failed
+ * request's status is published only after all of its launches have left
+ * the GPU).  SGEMM_OPERANDS_HOST: host memory; the server copies A, B (and C
+ * if beta != 0) into a device pool of hbm_budget_bytes, allocated by the
+ * first such request, on admission and C back on completion. */
 typedef struct {
     int m, n, k;
     float alpha, beta;
@@ -264,7 +283,9 @@ int sgemm_server_create(sgemm_server* srv, const sgemm_server_config* cfg);
  * 4*(mk + kn + mn) bytes exceeds the budget is rejected with
  * SGEMM_STATUS_BUDGET_EXCEEDED. */
 int sgemm_server_submit(sgemm_server srv, const sgemm_request* req, uint64_t* id);
-/* SGEMM_STATUS_NOT_READY while queued or in flight, then its final status. */
+/* SGEMM_STATUS_NOT_READY while queued or in flight, then its final status.
+ * The last 65536 finished requests stay queryable (poll, wait, latency);
+ * older ids return SGEMM_STATUS_NOT_FOUND. */
 int sgemm_server_poll(sgemm_server srv, uint64_t id);
 int sgemm_server_wait(sgemm_server srv, uint64_t id, double timeout_s);
 /* Wait until every submitted request has finished. */

@@ -63,10 +63,13 @@ int hip_status(hipError_t e) {
     return SGEMM_STATUS_HIP_ERROR;
 }
 
+}  // namespace
+
 // Shared body of every single-problem entry point.  config_id < 0 selects the
 // config from the autotune table.
-int run(int form, int config_id, int m, int n, int k, const float* alpha, const float* A, int lda,
-        const float* B, int ldb, const float* beta, float* C, int ldc, hipStream_t stream) {
+int sgemm_amd::run_problem(int form, int config_id, int m, int n, int k, const float* alpha,
+                           const float* A, int lda, const float* B, int ldb, const float* beta,
+                           float* C, int ldc, hipStream_t stream, const Arena* arena) {
     int st = validate(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     if (st != SGEMM_STATUS_SUCCESS) return st;
     if (form != SGEMM_FORM_F32 && form != SGEMM_FORM_BF16X3) return SGEMM_STATUS_NOT_SUPPORTED;
@@ -83,9 +86,20 @@ int run(int form, int config_id, int m, int n, int k, const float* alpha, const 
         return hip_status(launch_wide(p, stream));
     }
     if (config_id < 0) config_id = choose_config(form, m, n, k);
-    hipError_t e = form == SGEMM_FORM_F32 ? launch_f32(config_id, p, stream)
-                                          : launch_bf16x3(config_id, p, stream);
+    hipError_t e = form == SGEMM_FORM_F32 ? launch_f32(config_id, p, stream, arena)
+                                          : launch_bf16x3(config_id, p, stream, arena);
+    // With an arena nothing is allocated: out of memory means the plan needs
+    // more partials or counters than the arena holds (nothing was launched).
+    if (arena != nullptr && e == hipErrorOutOfMemory) return SGEMM_STATUS_SCRATCH_TOO_SMALL;
     return hip_status(e);
+}
+
+namespace {
+
+int run(int form, int config_id, int m, int n, int k, const float* alpha, const float* A, int lda,
+        const float* B, int ldb, const float* beta, float* C, int ldc, hipStream_t stream) {
+    return run_problem(form, config_id, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream,
+                       nullptr);
 }
 
 }  // namespace
@@ -135,6 +149,8 @@ const char* sgemm_status_string(int status) {
         case SGEMM_STATUS_NOT_FOUND: return "not found";
         case SGEMM_STATUS_IO_ERROR: return "I/O error";
         case SGEMM_STATUS_WRONG_STATE: return "wrong state";
+        case SGEMM_STATUS_SCRATCH_TOO_SMALL:
+            return "the problem needs more scratch than the static no-allocation array holds";
         default: return "unknown status";
     }
 }

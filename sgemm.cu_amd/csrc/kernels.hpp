@@ -20,6 +20,17 @@ struct ConfigInfo {
 // Compute units of the current device (queried once per process).
 int device_cu_count();
 
+// Where a launch takes its split-K partials and arrival counters from when
+// it is not the library's per-stream workspace: a fixed arena that never
+// grows (the no-allocation entry point's static array).  Its counters are
+// zero between launches, like the workspace's.
+struct Arena {
+    float* partials;
+    size_t partial_bytes;
+    int* counters;
+    int counter_count;
+};
+
 // Number of k-splits for a tiles-sized grid, each split keeping at least
 // min_ktiles k-tiles, at most 16.  BF16x3 (f32 = false): split only a grid of
 // fewer than two workgroups per CU, up to two per CU.  Plain FP32: the split
@@ -83,7 +94,8 @@ long long streamk_iters(long long rem, long long slots, long long iters, size_t 
 // partial round, one stream-K workgroup would suffice, or the stream is being
 // captured and its workspace would have to grow).
 hipError_t streamk_plan(hipStream_t stream, long long tiles, long long slots, int k, int bk,
-                        size_t split_bytes, double piece_iters, SplitK* sk, long long* nwg);
+                        size_t split_bytes, double piece_iters, SplitK* sk, long long* nwg,
+                        const Arena* arena = nullptr);
 // Plain FP32 rate of one CU running config 5's tile three at a time (141
 // TFLOP/s over 256 CUs at 8192^3).
 constexpr double kF32FlopsPerCuUs = 0.55e6;
@@ -92,12 +104,20 @@ constexpr double kF32FlopsPerCuUs = 0.55e6;
 // take split_bytes per tile: sets sk->splits (1 = no split) and kchunk, and
 // for a split grid the workspace pointers.
 hipError_t splitk_plan(hipStream_t stream, long long tiles, int k, int bk, size_t split_bytes,
-                       SplitK* sk, bool f32 = false, int min_ktiles = 4);
+                       SplitK* sk, bool f32 = false, int min_ktiles = 4,
+                       const Arena* arena = nullptr);
 
-// Per-stream split-K workspace (partials + zeroed per-tile counters), grown on
-// demand; the first call on a stream, or a larger problem, allocates.
+// Split-K partials + zeroed per-tile counters for a launch on `stream`.
+// arena == nullptr: the stream's workspace, grown on demand (the first call on
+// a stream, or a larger problem, allocates; a stream under capture returns
+// hipErrorStreamCaptureUnsupported instead of growing).  Otherwise the
+// arena's memory, or hipErrorOutOfMemory when the launch needs more than it
+// holds; nothing is allocated.
 hipError_t splitk_workspace(hipStream_t stream, size_t partial_bytes, int tiles,
-                            float** partials, int** counters);
+                            float** partials, int** counters, const Arena* arena = nullptr);
+
+// Frees the stream's workspace (after waiting for the stream).
+hipError_t splitk_workspace_release(hipStream_t stream);
 
 // Configs the persistent grouped kernel carries (grouped.hip).  Its
 // registers and LDS are the maximum over these, which sets its occupancy:
@@ -109,12 +129,14 @@ constexpr bool grouped_config(int form, int cfg) {
 // Plain FP32 configs (sgemm_f32.hip).
 int f32_config_count();
 const ConfigInfo& f32_config(int id);
-hipError_t launch_f32(int id, const Problem& p, hipStream_t stream);
+// arena: see splitk_workspace (nullptr: the stream's workspace).
+hipError_t launch_f32(int id, const Problem& p, hipStream_t stream, const Arena* arena = nullptr);
 
 // BF16x3 configs (sgemm_bf16x3.hip).
 int bf16x3_config_count();
 const ConfigInfo& bf16x3_config(int id);
-hipError_t launch_bf16x3(int id, const Problem& p, hipStream_t stream);
+hipError_t launch_bf16x3(int id, const Problem& p, hipStream_t stream,
+                         const Arena* arena = nullptr);
 
 // Naive one-thread-per-output kernel (sgemm_f32.hip).
 hipError_t launch_basic(const Problem& p, hipStream_t stream);
@@ -123,6 +145,15 @@ hipError_t launch_wide(const Problem& p, hipStream_t stream);
 
 // C := beta*C for the k == 0 / alpha == 0 shortcut (sgemm_f32.hip).
 hipError_t launch_scale_c(const Problem& p, hipStream_t stream);
+
+// The shared body of the single-problem entry points (api.cpp): validation,
+// the k == 0 / alpha == 0 shortcut, the wide-ld fallback and the config
+// launch; config_id < 0 picks from the autotune table.  Returns an
+// SGEMM_STATUS_* code; with an arena, SGEMM_STATUS_SCRATCH_TOO_SMALL when
+// the chosen plan needs more than the arena holds (nothing is launched).
+int run_problem(int form, int config_id, int m, int n, int k, const float* alpha,
+                const float* A, int lda, const float* B, int ldb, const float* beta, float* C,
+                int ldc, hipStream_t stream, const Arena* arena);
 
 inline int config_count(int form) {
     return form == kFormF32 ? f32_config_count() : form == kFormBf16x3 ? bf16x3_config_count() : 0;

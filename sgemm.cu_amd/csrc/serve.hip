@@ -27,23 +27,31 @@
 // Footprint = 4*(m*k + k*n + m*n) bytes (the request's operands and result),
 // reserved from admission to completion.
 //   * SGEMM_OPERANDS_HOST: the server owns the device copies.  It carves them
-//     out of one pool allocated at create time (first-fit, coalescing free
-//     list), copies A, B (and C when beta != 0) in on the slot's stream, and C
-//     back after the launch.
+//     out of one pool of the budget's size, allocated by the first such
+//     request (first-fit, coalescing free list), copies A, B (and C when
+//     beta != 0) in on the slot's stream, and C back after the launch.
 //   * SGEMM_OPERANDS_DEVICE: the operands already live on the device; the
 //     footprint is reserved against the budget while the request is in flight.
 //
 // Which lane, which panels and which config a request gets depend on its own
 // shape only, so its bits do not depend on which other requests share the GPU.
 //
+// A request's status is published once all of its work has left the GPU:
+// when one panel of a large request fails, its panels not yet launched are
+// dropped and the error is reported after the panels still in flight have
+// retired, so a caller that frees the operands on the error is safe.
+// Finished requests stay queryable (poll, wait, latency) for the last 65536
+// of them; the statistics are kept as running figures since the last reset.
+//
 // The executor is replaceable: SGEMM_EXECUTOR_MOCK completes a launch after
-// flops / mock_tflops seconds without touching a GPU, so admission, batching
-// and the statistics are testable on any host.
+// flops / mock_tflops seconds without touching a GPU, so admission, batching,
+// failure handling and the statistics are testable on any host.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -118,15 +126,17 @@ struct Req {
     sgemm_request r{};
     size_t footprint = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
-    int status = SGEMM_STATUS_NOT_READY;
+    int status = SGEMM_STATUS_NOT_READY;  // published only once the request is finished
+    int error = SGEMM_STATUS_SUCCESS;     // first failure seen while in flight
     bool pooled = false;
     size_t off[3] = {0, 0, 0};  // pool offsets of A, B, C (host operands)
     // Large lane: column panels of panel_n columns (the last may be narrower).
     bool large = false;
     int panels = 0, panel_n = 0, next_panel = 0, panels_done = 0;
-    bool panel_failed = false;
 };
 
+// A slot is the scheduler thread's own: only that thread reads or writes it
+// (the destructor runs after the thread has been joined).
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
@@ -135,11 +145,16 @@ struct Slot {
     Req* chunk = nullptr;      // large lane: the request one panel belongs to
     int chunk_panel = -1;
     bool busy = false;
+    bool finished = false;     // the launch has completed (seen by poll_slots)
+    bool failed = false;       // ... with an error
+    int launch_status = SGEMM_STATUS_SUCCESS;
     double mock_done_at = 0;
 };
 
 // Requests above this many flops go to the large lane (when chunking is on).
 constexpr double kDefaultChunkGflop = 100.0;
+// Finished requests whose status and latency stay queryable.
+constexpr size_t kHistory = 1 << 16;
 
 }  // namespace
 
@@ -153,13 +168,6 @@ class Server {
     int start() {
         t0_ = Clock::now();
         const bool gpu = cfg_.executor == SGEMM_EXECUTOR_GPU;
-        if (gpu) {
-            if (hipMalloc(&pool_, std::max<size_t>(cfg_.hbm_budget_bytes, 256)) != hipSuccess) {
-                (void)hipGetLastError();
-                pool_ = nullptr;
-                return SGEMM_STATUS_ALLOC_FAILED;
-            }
-        }
         slots_.resize(cfg_.num_slots);
         for (Slot& s : slots_) {
             if (!gpu) continue;
@@ -181,7 +189,12 @@ class Server {
         cv_.notify_all();
         if (worker_.joinable()) worker_.join();
         for (Slot& s : slots_) {
-            if (s.stream) (void)hipStreamSynchronize(s.stream);
+            if (s.stream) {
+                (void)hipStreamSynchronize(s.stream);
+                // Single-problem panels may have grown a split-K workspace
+                // for this stream; free it before the handle can be reused.
+                (void)sgemm_workspace_release(s.stream);
+            }
             if (s.plan) {
                 s.plan->release();
                 delete s.plan;
@@ -205,6 +218,12 @@ class Server {
             std::lock_guard<std::mutex> lock(mu_);
             ++rejected_;
             return SGEMM_STATUS_BUDGET_EXCEEDED;
+        }
+        // The device pool for host operands is allocated by the first request
+        // that needs it: a server fed only device operands holds none.
+        if (r.operands == SGEMM_OPERANDS_HOST) {
+            st = ensure_pool();
+            if (st != SGEMM_STATUS_SUCCESS) return st;
         }
         // Large lane: device operands above the chunk threshold, cut into
         // column panels of about chunk_flops_ each (multiples of 256 columns,
@@ -236,16 +255,19 @@ class Server {
 
     int wait(uint64_t id, double timeout_s) {
         std::unique_lock<std::mutex> lock(mu_);
-        auto it = all_.find(id);
-        if (it == all_.end()) return SGEMM_STATUS_NOT_FOUND;
-        Req* q = it->second.get();
         const auto deadline = Clock::now() + std::chrono::duration<double>(timeout_s);
-        while (q->status == SGEMM_STATUS_NOT_READY) {
-            if (done_cv_.wait_until(lock, deadline) == std::cv_status::timeout &&
-                q->status == SGEMM_STATUS_NOT_READY)
-                return SGEMM_STATUS_NOT_READY;
+        for (;;) {
+            // Looked up again after every wake-up: a finished request can
+            // leave the history while this thread sleeps.
+            auto it = all_.find(id);
+            if (it == all_.end()) return SGEMM_STATUS_NOT_FOUND;
+            if (it->second->status != SGEMM_STATUS_NOT_READY) return it->second->status;
+            if (done_cv_.wait_until(lock, deadline) == std::cv_status::timeout) {
+                it = all_.find(id);
+                if (it == all_.end()) return SGEMM_STATUS_NOT_FOUND;
+                return it->second->status;
+            }
         }
-        return q->status;
     }
 
     int drain(double timeout_s) {
@@ -270,28 +292,28 @@ class Server {
         return SGEMM_STATUS_SUCCESS;
     }
 
+    // Running figures since the last reset: the cost is one copy of the
+    // latencies measured since then, whatever the server's history.
     void stats(sgemm_server_stats* out) {
-        std::lock_guard<std::mutex> lock(mu_);
         std::memset(out, 0, sizeof(*out));
-        out->submitted = submitted_;
-        out->completed = completed_;
-        out->rejected = rejected_;
-        out->failed = failed_;
-        out->batches = batches_;
-        out->panel_launches = panel_launches_;
-        out->mean_batch = batches_ ? (double)batched_requests_ / (double)batches_ : 0.0;
-        out->peak_bytes = peak_bytes_;
-        out->budget_bytes = cfg_.hbm_budget_bytes;
         std::vector<double> lat;
-        double qsum = 0, flops = 0, first = 1e300, last = 0;
-        for (const auto& kv : all_) {
-            const Req* q = kv.second.get();
-            if (q->status != SGEMM_STATUS_SUCCESS || q->t_submit < stats_epoch_) continue;
-            lat.push_back((q->t_done - q->t_submit) * 1e3);
-            qsum += (q->t_admit - q->t_submit) * 1e3;
-            flops += 2.0 * q->r.m * q->r.n * (double)q->r.k;
-            first = std::min(first, q->t_submit);
-            last = std::max(last, q->t_done);
+        double qsum, first, last, flops;
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            out->submitted = submitted_;
+            out->completed = completed_;
+            out->rejected = rejected_;
+            out->failed = failed_;
+            out->batches = batches_;
+            out->panel_launches = panel_launches_;
+            out->mean_batch = batches_ ? (double)batched_requests_ / (double)batches_ : 0.0;
+            out->peak_bytes = peak_bytes_;
+            out->budget_bytes = cfg_.hbm_budget_bytes;
+            lat = lat_ms_;
+            qsum = queue_ms_sum_;
+            first = first_submit_;
+            last = last_done_;
+            flops = flops_sum_;
         }
         out->measured = (long long)lat.size();
         if (lat.empty()) return;
@@ -319,6 +341,11 @@ class Server {
         batches_ = 0;
         batched_requests_ = 0;
         panel_launches_ = 0;
+        lat_ms_.clear();
+        queue_ms_sum_ = 0;
+        flops_sum_ = 0;
+        first_submit_ = 1e300;
+        last_done_ = 0;
     }
 
   private:
@@ -326,12 +353,53 @@ class Server {
         return 4 * ((size_t)r.m * r.k + (size_t)r.k * r.n + (size_t)r.m * r.n);
     }
 
-    // Scheduler thread.
+    int ensure_pool() {
+        if (cfg_.executor != SGEMM_EXECUTOR_GPU) return SGEMM_STATUS_SUCCESS;  // offsets only
+        std::lock_guard<std::mutex> lock(pool_mu_);
+        if (pool_) return SGEMM_STATUS_SUCCESS;
+        if (hipMalloc(&pool_, std::max<size_t>(cfg_.hbm_budget_bytes, 256)) != hipSuccess) {
+            (void)hipGetLastError();
+            pool_ = nullptr;
+            return SGEMM_STATUS_ALLOC_FAILED;
+        }
+        return SGEMM_STATUS_SUCCESS;
+    }
+
+    // Scheduler thread.  Completion checks and launches run without mu_:
+    // event queries, descriptor uploads, kernel launches and pageable copies
+    // can take a while, and submit(), poll() and stats() must not wait
+    // behind them.  What they touch is the scheduler's own (its slots) or
+    // immutable while a request is in flight (its operands, panel layout and
+    // pool offsets), and in-flight requests never leave all_.
     void loop() {
         std::unique_lock<std::mutex> lock(mu_);
+        std::vector<Slot*> planned;
         while (!stop_) {
-            retire();
-            admit();
+            lock.unlock();
+            const bool any_done = poll_slots();
+            lock.lock();
+            if (any_done) retire();
+            planned.clear();
+            admit(&planned);
+            if (!planned.empty()) {
+                lock.unlock();
+                for (Slot* s : planned) {
+                    s->launch_status = s->chunk ? launch_chunk(*s) : launch(*s);
+                    if (s->launch_status != SGEMM_STATUS_SUCCESS) {
+                        // The slot retires once whatever was enqueued before
+                        // the failure has drained; if even this record
+                        // fails, the event still holds an earlier (completed)
+                        // record.
+                        s->mock_done_at = 0;
+                        if (cfg_.executor == SGEMM_EXECUTOR_GPU)
+                            (void)hipEventRecord(s->done, s->stream);
+                    }
+                }
+                lock.lock();
+                for (Slot* s : planned)
+                    if (s->launch_status != SGEMM_STATUS_SUCCESS) launch_failed(*s);
+                continue;  // more may be admissible now
+            }
             bool any_busy = false;
             for (const Slot& s : slots_) any_busy |= s.busy;
             if (any_busy)
@@ -339,59 +407,91 @@ class Server {
             else
                 cv_.wait_for(lock, std::chrono::milliseconds(50));
         }
-        // Let in-flight work finish before the slots are torn down.
-        for (Slot& s : slots_)
-            if (s.busy && s.done) {
-                lock.unlock();
-                (void)hipEventSynchronize(s.done);  // an error here leaves nothing to wait for
-                lock.lock();
-            }
     }
 
-    bool slot_done(Slot& s) {
-        if (cfg_.executor == SGEMM_EXECUTOR_MOCK) return seconds_since(t0_) >= s.mock_done_at;
-        hipError_t e = hipEventQuery(s.done);
-        if (e == hipErrorNotReady) return false;
-        if (e != hipSuccess) {
-            for (Req* q : s.batch) q->status = SGEMM_STATUS_HIP_ERROR;
-            if (s.chunk) s.chunk->status = SGEMM_STATUS_HIP_ERROR;
+    // Marks busy slots whose launch has completed (no lock held).
+    bool poll_slots() {
+        bool any = false;
+        const bool mock = cfg_.executor == SGEMM_EXECUTOR_MOCK;
+        const double now = mock ? seconds_since(t0_) : 0.0;
+        for (Slot& s : slots_) {
+            if (!s.busy || s.finished) continue;
+            if (mock) {
+                if (now < s.mock_done_at) continue;
+            } else {
+                const hipError_t e = hipEventQuery(s.done);
+                if (e == hipErrorNotReady) continue;
+                if (e != hipSuccess) {
+                    (void)hipGetLastError();
+                    s.failed = true;
+                }
+            }
+            s.finished = true;
+            any = true;
         }
-        return true;
+        return any;
+    }
+
+    // A launch returned an error (mu_ held): its requests fail once the slot
+    // retires.  For a panel, the request's panels that were never launched
+    // are dropped (counted as done), so the request finishes -- with the
+    // error -- when its panels already in flight on other slots have retired,
+    // not before: until then the caller must not free its operands.
+    void launch_failed(Slot& s) {
+        const int err = s.launch_status;
+        if (Req* q = s.chunk) {
+            if (q->error == SGEMM_STATUS_SUCCESS) q->error = err;
+            q->panels_done += q->panels - q->next_panel;
+            q->next_panel = q->panels;
+            return;
+        }
+        for (Req* q : s.batch)
+            if (q->error == SGEMM_STATUS_SUCCESS) q->error = err;
     }
 
     void finish(Req* q, double now) {
         q->t_done = now;
-        if (q->status == SGEMM_STATUS_NOT_READY) {
-            q->status = SGEMM_STATUS_SUCCESS;
+        q->status = q->error == SGEMM_STATUS_SUCCESS ? SGEMM_STATUS_SUCCESS : q->error;
+        if (q->status == SGEMM_STATUS_SUCCESS) {
             ++completed_;
+            if (q->t_submit >= stats_epoch_) {
+                lat_ms_.push_back((q->t_done - q->t_submit) * 1e3);
+                queue_ms_sum_ += (q->t_admit - q->t_submit) * 1e3;
+                flops_sum_ += 2.0 * q->r.m * q->r.n * (double)q->r.k;
+                first_submit_ = std::min(first_submit_, q->t_submit);
+                last_done_ = std::max(last_done_, q->t_done);
+            }
         } else {
             ++failed_;
         }
         release(q);
+        finished_ids_.push_back(q->id);
+        while (finished_ids_.size() > kHistory) {
+            all_.erase(finished_ids_.front());
+            finished_ids_.pop_front();
+        }
     }
 
     void retire() {
         bool any = false;
+        const double now = seconds_since(t0_);
         for (Slot& s : slots_) {
-            if (!s.busy || !slot_done(s)) continue;
-            const double now = seconds_since(t0_);
+            if (!s.busy || !s.finished) continue;
             if (Req* q = s.chunk) {
                 ++q->panels_done;
-                if (q->status != SGEMM_STATUS_NOT_READY) q->panel_failed = true;
-                if (q->panels_done == q->panels) {
-                    if (q->panel_failed && q->status == SGEMM_STATUS_NOT_READY)
-                        q->status = SGEMM_STATUS_HIP_ERROR;
-                    finish(q, now);
-                }
+                if (s.failed && q->error == SGEMM_STATUS_SUCCESS) q->error = SGEMM_STATUS_HIP_ERROR;
+                if (q->panels_done == q->panels) finish(q, now);
                 s.chunk = nullptr;
                 s.chunk_panel = -1;
-                s.busy = false;
-                any = true;
-                continue;
+            } else {
+                for (Req* q : s.batch) {
+                    if (s.failed && q->error == SGEMM_STATUS_SUCCESS)
+                        q->error = SGEMM_STATUS_HIP_ERROR;
+                    finish(q, now);
+                }
+                s.batch.clear();
             }
-            for (Req* q : s.batch) finish(q, now);
-            s.batch.clear();
-            s.busy = false;
+            s.busy = s.finished = s.failed = false;
             any = true;
         }
         if (any) done_cv_.notify_all();
@@ -425,18 +525,21 @@ class Server {
         return true;
     }
 
-    void admit() {
+    // Picks work for the free slots (mu_ held); the launches follow without
+    // the lock.
+    void admit(std::vector<Slot*>* planned) {
         for (Slot& s : slots_) {
             if (s.busy) continue;
             const bool large_first = prefer_large_;
-            if (!(large_first ? launch_panel(s) || launch_small(s)
-                              : launch_small(s) || launch_panel(s)))
+            if (!(large_first ? plan_panel(s) || plan_small(s)
+                              : plan_small(s) || plan_panel(s)))
                 return;  // nothing admissible for any slot
+            planned->push_back(&s);
         }
     }
 
     // Small lane: a grouped launch of up to max_batch requests from the head.
-    bool launch_small(Slot& s) {
+    bool plan_small(Slot& s) {
         // A large request waiting for memory keeps its footprint free: small
         // requests may overtake it, but never push it out of the budget.
         const size_t headroom =
@@ -450,24 +553,17 @@ class Server {
             batch.push_back(q);
         }
         if (batch.empty()) return false;
-        s.batch = batch;
+        s.batch = std::move(batch);
         s.busy = true;
         ++batches_;
-        batched_requests_ += (long long)batch.size();
+        batched_requests_ += (long long)s.batch.size();
         prefer_large_ = true;
-        if (launch(s) != SGEMM_STATUS_SUCCESS) {
-            for (Req* q : s.batch) q->status = SGEMM_STATUS_HIP_ERROR;
-            s.mock_done_at = 0;  // retire at once
-            // The slot retires when s.done completes; if even this record
-            // fails, the event still holds an earlier (completed) record.
-            if (cfg_.executor == SGEMM_EXECUTOR_GPU) (void)hipEventRecord(s.done, s.stream);
-        }
         return true;
     }
 
     // Large lane: the next panel of the active large request, admitting the
     // next one when the active request has launched all of its panels.
-    bool launch_panel(Slot& s) {
+    bool plan_panel(Slot& s) {
         if (large_active_ && large_active_->next_panel == large_active_->panels)
             large_active_ = nullptr;
         if (!large_active_) {
@@ -484,12 +580,13 @@ class Server {
         s.busy = true;
         ++panel_launches_;
         prefer_large_ = false;
-        if (launch_chunk(s) != SGEMM_STATUS_SUCCESS) {
-            q->status = SGEMM_STATUS_HIP_ERROR;
-            s.mock_done_at = 0;
-            if (cfg_.executor == SGEMM_EXECUTOR_GPU) (void)hipEventRecord(s.done, s.stream);  // as above
-        }
         return true;
+    }
+
+    // Mock executor: the launch with (1-based) index cfg_.mock_fail_launch
+    // fails, to exercise the failure paths without a GPU.
+    bool mock_fails() {
+        return cfg_.mock_fail_launch > 0 && ++mock_launches_ == cfg_.mock_fail_launch;
     }
 
     // One column panel as a single-problem launch (best config for its shape).
@@ -498,6 +595,7 @@ class Server {
         const int j0 = s.chunk_panel * s.chunk->panel_n;
         const int nc = std::min(s.chunk->panel_n, r.n - j0);
         if (cfg_.executor == SGEMM_EXECUTOR_MOCK) {
+            if (mock_fails()) return SGEMM_STATUS_HIP_ERROR;
             s.mock_done_at = seconds_since(t0_) + cfg_.mock_launch_us * 1e-6 +
                              2.0 * r.m * nc * (double)r.k / (cfg_.mock_tflops * 1e12);
             return SGEMM_STATUS_SUCCESS;
@@ -514,18 +612,16 @@ class Server {
         if (sgemm_autotune_lookup(cfg_.form, 0, r.m, nc, r.k, &cfg, &from_table) ==
             SGEMM_STATUS_SUCCESS)
             whole = whole_tile_twin(cfg_.form, cfg);
-        if (whole >= 0 && r.lda <= kMaxLd && r.ldb <= kMaxLd && r.ldc <= kMaxLd) {
-            const int st = sgemm_run_config(cfg_.form, whole, r.m, nc, r.k, &r.alpha, r.A,
-                                            r.lda, B, r.ldb, &r.beta, C, r.ldc, s.stream);
-            if (st != SGEMM_STATUS_SUCCESS) return st;
-            return hipEventRecord(s.done, s.stream) == hipSuccess ? SGEMM_STATUS_SUCCESS
-                                                                  : SGEMM_STATUS_HIP_ERROR;
-        }
-        const int st = cfg_.form == SGEMM_FORM_BF16X3
-                           ? sgemm_bf16x3(r.m, nc, r.k, &r.alpha, r.A, r.lda, B, r.ldb, &r.beta,
-                                          C, r.ldc, s.stream)
-                           : sgemm_ex(r.m, nc, r.k, &r.alpha, r.A, r.lda, B, r.ldb, &r.beta, C,
-                                      r.ldc, s.stream);
+        int st;
+        if (whole >= 0 && r.lda <= kMaxLd && r.ldb <= kMaxLd && r.ldc <= kMaxLd)
+            st = sgemm_run_config(cfg_.form, whole, r.m, nc, r.k, &r.alpha, r.A, r.lda, B, r.ldb,
+                                  &r.beta, C, r.ldc, s.stream);
+        else
+            st = cfg_.form == SGEMM_FORM_BF16X3
+                     ? sgemm_bf16x3(r.m, nc, r.k, &r.alpha, r.A, r.lda, B, r.ldb, &r.beta, C,
+                                    r.ldc, s.stream)
+                     : sgemm_ex(r.m, nc, r.k, &r.alpha, r.A, r.lda, B, r.ldb, &r.beta, C, r.ldc,
+                                s.stream);
         if (st != SGEMM_STATUS_SUCCESS) return st;
         return hipEventRecord(s.done, s.stream) == hipSuccess ? SGEMM_STATUS_SUCCESS
                                                               : SGEMM_STATUS_HIP_ERROR;
@@ -535,6 +631,7 @@ class Server {
         double flops = 0;
         for (Req* q : s.batch) flops += 2.0 * q->r.m * q->r.n * (double)q->r.k;
         if (cfg_.executor == SGEMM_EXECUTOR_MOCK) {
+            if (mock_fails()) return SGEMM_STATUS_HIP_ERROR;
             s.mock_done_at = seconds_since(t0_) + cfg_.mock_launch_us * 1e-6 +
                              flops / (cfg_.mock_tflops * 1e12);
             return SGEMM_STATUS_SUCCESS;
@@ -589,7 +686,8 @@ class Server {
 
     sgemm_server_config cfg_;
     RangeAllocator alloc_;
-    void* pool_ = nullptr;
+    std::mutex pool_mu_;
+    void* pool_ = nullptr;       // host-operand pool, allocated on first use
     std::vector<Slot> slots_;
     std::deque<Req*> pending_;   // small lane, arrival order
     std::deque<Req*> large_q_;   // large lane, arrival order
@@ -597,7 +695,9 @@ class Server {
     bool prefer_large_ = false;
     double chunk_flops_ = 0;
     long long panel_launches_ = 0;
+    int mock_launches_ = 0;      // scheduler thread only
     std::unordered_map<uint64_t, std::unique_ptr<Req>> all_;
+    std::deque<uint64_t> finished_ids_;  // oldest first; evicted past kHistory
     std::mutex mu_;
     std::condition_variable cv_, done_cv_;
     std::thread worker_;
@@ -608,6 +708,9 @@ class Server {
     long long batches_ = 0, batched_requests_ = 0;
     size_t used_bytes_ = 0, peak_bytes_ = 0;
     double stats_epoch_ = 0;
+    // Running statistics of the requests submitted since stats_epoch_.
+    std::vector<double> lat_ms_;
+    double queue_ms_sum_ = 0, flops_sum_ = 0, first_submit_ = 1e300, last_done_ = 0;
 };
 
 }  // namespace sgemm_amd

@@ -80,7 +80,7 @@ int bf16x3_config_count() { return (int)(sizeof(kBf16x3Configs) / sizeof(kBf16x3
 const ConfigInfo& bf16x3_config(int id) { return kBf16x3Configs[id]; }
 
 template <class T, bool ALLOW_SPLIT, int MIN_KTILES = 4>
-static hipError_t launch_x3(const Problem& p, hipStream_t s) {
+static hipError_t launch_x3(const Problem& p, hipStream_t s, const Arena* arena) {
     Problem q = p;
     q.tiles_m = div_up(p.m, T::BM);
     q.tiles_n = div_up(p.n, T::BN);
@@ -92,7 +92,7 @@ static hipError_t launch_x3(const Problem& p, hipStream_t s) {
     if (ALLOW_SPLIT) {
         constexpr size_t acc_bytes = T::MF == 16 ? 16 : 64;
         hipError_t e = splitk_plan(s, tiles, p.k, T::BK, T::MB * T::NB * acc_bytes * T::NT, &sk,
-                                   false, MIN_KTILES);
+                                   false, MIN_KTILES, arena);
         if (e != hipSuccess) return e;
     }
     const bool split = ALLOW_SPLIT && sk.splits > 1;
@@ -131,7 +131,7 @@ static long long x3_resident_workgroups() {
 // in a full one, so without the split a round with a few tiles costs as much
 // as a full one.
 template <class T>
-static hipError_t launch_x3_tail(const Problem& p, hipStream_t s) {
+static hipError_t launch_x3_tail(const Problem& p, hipStream_t s, const Arena* arena) {
     Problem q = p;
     q.tiles_m = div_up(p.m, T::BM);
     q.tiles_n = div_up(p.n, T::BN);
@@ -154,7 +154,7 @@ static hipError_t launch_x3_tail(const Problem& p, hipStream_t s) {
         sk.splits = div_up(p.k, sk.kchunk);
         if (sk.splits > 1) {
             const hipError_t e = splitk_workspace(s, (size_t)rem * sk.splits * split_bytes,
-                                                  (int)rem, &sk.partials, &sk.counters);
+                                                  (int)rem, &sk.partials, &sk.counters, arena);
             if (e == hipErrorStreamCaptureUnsupported) {
                 sk.splits = 1;
                 sk.kchunk = p.k;
@@ -185,7 +185,7 @@ static hipError_t launch_x3_tail(const Problem& p, hipStream_t s) {
 // there is no partial round, or when the stream is being captured and its
 // workspace would have to grow.
 template <class T>
-static hipError_t launch_x3_streamk(const Problem& p, hipStream_t s) {
+static hipError_t launch_x3_streamk(const Problem& p, hipStream_t s, const Arena* arena) {
     Problem q = p;
     q.tiles_m = div_up(p.m, T::BM);
     q.tiles_n = div_up(p.n, T::BN);
@@ -200,7 +200,8 @@ static hipError_t launch_x3_streamk(const Problem& p, hipStream_t s) {
     SplitK sk{nullptr, nullptr, 1, p.k, 0};
     long long nwg = 0;
     const hipError_t e =
-        streamk_plan(s, tiles, slots, p.k, T::BK, split_bytes, kX3StreamKPieceIters, &sk, &nwg);
+        streamk_plan(s, tiles, slots, p.k, T::BK, split_bytes, kX3StreamKPieceIters, &sk, &nwg,
+                     arena);
     if (e != hipSuccess) return e;
     (void)hipGetLastError();
     const bool vec = vec4_ok(p);
@@ -217,18 +218,20 @@ static hipError_t launch_x3_streamk(const Problem& p, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_bf16x3(int id, const Problem& p, hipStream_t s) {
+hipError_t launch_bf16x3(int id, const Problem& p, hipStream_t s, const Arena* arena) {
     if (id < 0 || id >= bf16x3_config_count()) return hipErrorInvalidValue;
     switch (id) {
-        case 6: return launch_x3<X3_128x128, true>(p, s);
-        case 7: return launch_x3<X3_64x64, true>(p, s);
-        case 9: return launch_x3_tail<X3_256x256_w8>(p, s);
-        case 10: return launch_x3<X3_128x128_2cu, true, kX3WideSplitKtiles>(p, s);
-        case 11: return launch_x3_streamk<X3_256x256_w8>(p, s);
+        case 6: return launch_x3<X3_128x128, true>(p, s, arena);
+        case 7: return launch_x3<X3_64x64, true>(p, s, arena);
+        case 9: return launch_x3_tail<X3_256x256_w8>(p, s, arena);
+        case 10: return launch_x3<X3_128x128_2cu, true, kX3WideSplitKtiles>(p, s, arena);
+        case 11: return launch_x3_streamk<X3_256x256_w8>(p, s, arena);
         default: break;
     }
     hipError_t e = hipErrorInvalidValue;
-    dispatch_x3(id, [&](auto tag) { e = launch_x3<typename decltype(tag)::type, false>(p, s); });
+    dispatch_x3(id, [&](auto tag) {
+        e = launch_x3<typename decltype(tag)::type, false>(p, s, arena);
+    });
     return e;
 }
 

@@ -84,7 +84,7 @@ int f32_config_count() { return (int)(sizeof(kF32Configs) / sizeof(kF32Configs[0
 const ConfigInfo& f32_config(int id) { return kF32Configs[id]; }
 
 template <class T, bool ALLOW_SPLIT, int MIN_KTILES = 4>
-static hipError_t launch_tile(const Problem& p, hipStream_t s) {
+static hipError_t launch_tile(const Problem& p, hipStream_t s, const Arena* arena) {
     Problem q = p;
     q.tiles_m = div_up(p.m, T::BM);
     q.tiles_n = div_up(p.n, T::BN);
@@ -96,7 +96,7 @@ static hipError_t launch_tile(const Problem& p, hipStream_t s) {
     if (ALLOW_SPLIT) {
         hipError_t e =
             splitk_plan(s, tiles, p.k, T::BK, (size_t)T::MB * T::NB * T::NT * 16, &sk, true,
-                        MIN_KTILES);
+                        MIN_KTILES, arena);
         if (e != hipSuccess) return e;
     }
     const bool split = ALLOW_SPLIT && sk.splits > 1;
@@ -135,7 +135,7 @@ static long long resident_workgroups() {
 // round, when the split would not pay, or when the stream is being captured
 // and its workspace would have to grow.
 template <class T>
-static hipError_t launch_tail_split(const Problem& p, hipStream_t s) {
+static hipError_t launch_tail_split(const Problem& p, hipStream_t s, const Arena* arena) {
     Problem q = p;
     q.tiles_m = div_up(p.m, T::BM);
     q.tiles_n = div_up(p.n, T::BN);
@@ -157,7 +157,7 @@ static hipError_t launch_tail_split(const Problem& p, hipStream_t s) {
         sk.splits = div_up(p.k, sk.kchunk);
         if (sk.splits > 1) {
             const hipError_t e = splitk_workspace(s, (size_t)rem * sk.splits * split_bytes,
-                                                  (int)rem, &sk.partials, &sk.counters);
+                                                  (int)rem, &sk.partials, &sk.counters, arena);
             if (e == hipErrorStreamCaptureUnsupported) {
                 sk.splits = 1;
                 sk.kchunk = p.k;
@@ -188,7 +188,7 @@ static hipError_t launch_tail_split(const Problem& p, hipStream_t s) {
 // is no partial round, or when the stream is being captured and its
 // workspace would have to grow.
 template <class T>
-static hipError_t launch_streamk(const Problem& p, hipStream_t s) {
+static hipError_t launch_streamk(const Problem& p, hipStream_t s, const Arena* arena) {
     Problem q = p;
     q.tiles_m = div_up(p.m, T::BM);
     q.tiles_n = div_up(p.n, T::BN);
@@ -202,7 +202,8 @@ static hipError_t launch_streamk(const Problem& p, hipStream_t s) {
     SplitK sk{nullptr, nullptr, 1, p.k, 0};
     long long nwg = 0;
     const hipError_t e =
-        streamk_plan(s, tiles, slots, p.k, T::BK, split_bytes, kF32StreamKPieceIters, &sk, &nwg);
+        streamk_plan(s, tiles, slots, p.k, T::BK, split_bytes, kF32StreamKPieceIters, &sk, &nwg,
+                     arena);
     if (e != hipSuccess) return e;
     (void)hipGetLastError();
     const bool vec = vec4_ok(p);
@@ -219,20 +220,22 @@ static hipError_t launch_streamk(const Problem& p, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_f32(int id, const Problem& p, hipStream_t s) {
+hipError_t launch_f32(int id, const Problem& p, hipStream_t s, const Arena* arena) {
     if (id < 0 || id >= f32_config_count()) return hipErrorInvalidValue;
     switch (id) {
-        case 6: return launch_tile<F32_128x128, true>(p, s);
-        case 7: return launch_tile<F32_64x64, true>(p, s);
-        case 8: return launch_tile<F32_128x64, true>(p, s);
-        case 9: return launch_tail_split<F32_128x128_k16>(p, s);
-        case 10: return launch_tile<F32_128x128_k16, true, 8>(p, s);
-        case 11: return launch_streamk<F32_128x128_k16>(p, s);
-        case 12: return launch_tile<F32_64x64_kg2, false>(p, s);
+        case 6: return launch_tile<F32_128x128, true>(p, s, arena);
+        case 7: return launch_tile<F32_64x64, true>(p, s, arena);
+        case 8: return launch_tile<F32_128x64, true>(p, s, arena);
+        case 9: return launch_tail_split<F32_128x128_k16>(p, s, arena);
+        case 10: return launch_tile<F32_128x128_k16, true, 8>(p, s, arena);
+        case 11: return launch_streamk<F32_128x128_k16>(p, s, arena);
+        case 12: return launch_tile<F32_64x64_kg2, false>(p, s, arena);
         default: break;
     }
     hipError_t e = hipErrorInvalidValue;
-    dispatch_f32(id, [&](auto tag) { e = launch_tile<typename decltype(tag)::type, false>(p, s); });
+    dispatch_f32(id, [&](auto tag) {
+        e = launch_tile<typename decltype(tag)::type, false>(p, s, arena);
+    });
     return e;
 }
 

@@ -9,10 +9,17 @@
 // captured graph).  A stream that is being captured never allocates: a
 // split-K launch whose workspace would have to grow runs unsplit instead, so
 // warm a stream up before capturing on it to keep the split.  A workspace
-// that is outgrown is retired, not freed: a
-// captured graph may still hold its address, so it stays valid until the
-// process ends.  Growth is geometric, so the retired total stays below the
-// live workspace.
+// that is outgrown is retired, not freed: a captured graph may still hold its
+// address, so it stays valid until the stream's workspace is released
+// (splitk_workspace_release) or the process ends.  Growth is geometric, so
+// the retired total stays below the live workspace.
+//
+// Launches that share a workspace must not overlap: eager calls on one
+// stream are ordered by the stream, but a graph that captured split launches
+// on that stream uses the same partials and counters when it is replayed, so
+// its replays must be ordered with each other and with eager calls on the
+// capture stream (a counter left non-zero by an overlapping launch makes a
+// later launch skip writing tiles).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -20,6 +27,7 @@
 #include <mutex>
 #include <vector>
 
+#include "sgemm.h"
 #include "kernels.hpp"
 
 namespace sgemm_amd {
@@ -31,11 +39,11 @@ struct Workspace {
     size_t partial_bytes = 0;
     int* counters = nullptr;
     int counter_count = 0;
+    std::vector<void*> retired;  // outgrown buffers, kept for captured graphs
 };
 
 std::mutex g_mu;
 std::map<hipStream_t, Workspace> g_ws;
-std::vector<void*> g_retired;  // outgrown buffers, kept for captured graphs
 
 }  // namespace
 
@@ -125,7 +133,8 @@ long long streamk_iters(long long rem, long long slots, long long iters, size_t 
 }
 
 hipError_t streamk_plan(hipStream_t stream, long long tiles, long long slots, int k, int bk,
-                        size_t split_bytes, double piece_iters, SplitK* sk, long long* nwg) {
+                        size_t split_bytes, double piece_iters, SplitK* sk, long long* nwg,
+                        const Arena* arena) {
     *nwg = 0;
     const long long rem = slots > 0 ? tiles % slots : 0;
     if (rem <= 0 || k <= 0) return hipSuccess;
@@ -138,7 +147,7 @@ hipError_t streamk_plan(hipStream_t stream, long long tiles, long long slots, in
     sk->iters = (int)w;
     sk->pieces = (int)((iters - 1) / w + 2);
     const hipError_t e = splitk_workspace(stream, (size_t)n * sk->pieces * split_bytes, (int)n,
-                                          &sk->partials, &sk->counters);
+                                          &sk->partials, &sk->counters, arena);
     if (e == hipErrorStreamCaptureUnsupported) return hipSuccess;
     if (e != hipSuccess) return e;
     *nwg = n;
@@ -146,7 +155,7 @@ hipError_t streamk_plan(hipStream_t stream, long long tiles, long long slots, in
 }
 
 hipError_t splitk_plan(hipStream_t stream, long long tiles, int k, int bk, size_t split_bytes,
-                       SplitK* sk, bool f32, int min_ktiles) {
+                       SplitK* sk, bool f32, int min_ktiles, const Arena* arena) {
     sk->partials = nullptr;
     sk->counters = nullptr;
     sk->splits = splitk_count(tiles, k, bk, f32, min_ktiles);
@@ -154,7 +163,7 @@ hipError_t splitk_plan(hipStream_t stream, long long tiles, int k, int bk, size_
     sk->splits = div_up(k, sk->kchunk);  // every split gets some k
     if (sk->splits <= 1) return hipSuccess;
     const hipError_t e = splitk_workspace(stream, (size_t)tiles * sk->splits * split_bytes,
-                                          (int)tiles, &sk->partials, &sk->counters);
+                                          (int)tiles, &sk->partials, &sk->counters, arena);
     if (e == hipErrorStreamCaptureUnsupported) {
         // Capturing, and this stream's workspace is too small: run unsplit
         // rather than allocate inside the capture.
@@ -168,7 +177,14 @@ hipError_t splitk_plan(hipStream_t stream, long long tiles, int k, int bk, size_
 }
 
 hipError_t splitk_workspace(hipStream_t stream, size_t partial_bytes, int tiles,
-                            float** partials, int** counters) {
+                            float** partials, int** counters, const Arena* arena) {
+    if (arena != nullptr) {
+        if (partial_bytes > arena->partial_bytes || tiles > arena->counter_count)
+            return hipErrorOutOfMemory;
+        *partials = arena->partials;
+        *counters = arena->counters;
+        return hipSuccess;
+    }
     std::lock_guard<std::mutex> lock(g_mu);
     Workspace& w = g_ws[stream];
     if (w.partial_bytes < partial_bytes || w.counter_count < tiles) {
@@ -183,7 +199,7 @@ hipError_t splitk_workspace(hipStream_t stream, size_t partial_bytes, int tiles,
             (void)hipGetLastError();
             return hipErrorOutOfMemory;
         }
-        if (w.partials) g_retired.push_back(w.partials);
+        if (w.partials) w.retired.push_back(w.partials);
         w.partials = p;
         w.partial_bytes = bytes;
     }
@@ -203,7 +219,7 @@ hipError_t splitk_workspace(hipStream_t stream, size_t partial_bytes, int tiles,
             (void)hipFree(c);
             return hipErrorUnknown;
         }
-        if (w.counters) g_retired.push_back(w.counters);
+        if (w.counters) w.retired.push_back(w.counters);
         w.counters = c;
         w.counter_count = n;
     }
@@ -212,4 +228,28 @@ hipError_t splitk_workspace(hipStream_t stream, size_t partial_bytes, int tiles,
     return hipSuccess;
 }
 
+hipError_t splitk_workspace_release(hipStream_t stream) {
+    Workspace w;
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        auto it = g_ws.find(stream);
+        if (it == g_ws.end()) return hipSuccess;
+        w = std::move(it->second);
+        g_ws.erase(it);
+    }
+    // The stream's pending launches may still use the buffers.
+    hipError_t e = hipStreamSynchronize(stream);
+    if (w.partials) (void)hipFree(w.partials);
+    if (w.counters) (void)hipFree(w.counters);
+    for (void* p : w.retired) (void)hipFree(p);
+    return e;
+}
+
 }  // namespace sgemm_amd
+
+extern "C" int sgemm_workspace_release(sgemm_stream_t stream) {
+    const hipError_t e = sgemm_amd::splitk_workspace_release(static_cast<hipStream_t>(stream));
+    if (e == hipSuccess) return SGEMM_STATUS_SUCCESS;
+    (void)hipGetLastError();
+    return SGEMM_STATUS_HIP_ERROR;
+}

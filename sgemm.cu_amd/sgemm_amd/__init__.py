@@ -57,7 +57,7 @@ class ServerConfig(ctypes.Structure):
         ("hbm_budget_bytes", ctypes.c_size_t), ("max_batch", ctypes.c_int),
         ("num_slots", ctypes.c_int), ("form", ctypes.c_int), ("executor", ctypes.c_int),
         ("mock_tflops", ctypes.c_double), ("mock_launch_us", ctypes.c_double),
-        ("chunk_gflop", ctypes.c_double),
+        ("chunk_gflop", ctypes.c_double), ("mock_fail_launch", ctypes.c_int),
     ]
 
 
@@ -177,6 +177,7 @@ def lib() -> ctypes.CDLL:
         "sgemm_server_get_stats": [plan_p, ctypes.POINTER(ServerStats)],
         "sgemm_server_reset_stats": [plan_p],
         "sgemm_server_destroy": [plan_p],
+        "sgemm_workspace_release": [vp],
     }.items():
         getattr(L, name).argtypes = args
         getattr(L, name).restype = i
@@ -265,6 +266,11 @@ def run_config(form, config_id, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st
     if check:
         _check(st, "sgemm_run_config")
     return st
+
+
+def workspace_release(stream=None) -> None:
+    """Free the library's split-K workspace of `stream` (sgemm_workspace_release)."""
+    _check(lib().sgemm_workspace_release(_stream(stream)), "sgemm_workspace_release")
 
 
 def configs(form: int) -> list[dict]:
@@ -398,9 +404,10 @@ class Server:
     """Continuous-batching GEMM server (sgemm_server_* in include/sgemm.h)."""
 
     def __init__(self, budget_bytes, max_batch=64, num_slots=2, form=FORM_BF16X3,
-                 executor=EXECUTOR_GPU, mock_tflops=0.0, mock_launch_us=0.0, chunk_gflop=0.0):
+                 executor=EXECUTOR_GPU, mock_tflops=0.0, mock_launch_us=0.0, chunk_gflop=0.0,
+                 mock_fail_launch=0):
         cfg = ServerConfig(budget_bytes, max_batch, num_slots, form, executor, mock_tflops,
-                           mock_launch_us, chunk_gflop)
+                           mock_launch_us, chunk_gflop, mock_fail_launch)
         self.handle = ctypes.c_void_p()
         _check(lib().sgemm_server_create(ctypes.byref(self.handle), ctypes.byref(cfg)),
                "sgemm_server_create")

@@ -127,6 +127,44 @@ def test_wide_range_error_report(n):
         f"{res[1]['max_rel']:.3e}/{res[1]['mean_rel']:.3e}")
 
 
+@pytest.mark.parametrize("n", [8192, 16384])
+def test_error_against_plain_fp32_same_entries(n):
+    """Both forms on the same wide-range inputs (sign * 2^u, u in [-8, 8)) at
+    the split-precision benchmark's sizes, against the host FP64 oracle on
+    the same 8192 sampled entries.
+
+    The benchmark's target -- BF16x3 error no larger than the plain form's --
+    is NOT met on these entries, and this test states by how much: the
+    two-term split drops a residual of ~2^-17 |x| per operand, which puts the
+    BF16x3 mean componentwise error at 2-3.5x the plain kernel's and its
+    sampled max at about 1.1-1.4x (measured on MI355X: 8192^3 max 1.06e-6 vs
+    0.84e-6, mean 2.0e-7 vs 6.6e-8; 16384^3 max 6.8e-7 vs 5.7e-7, mean 1.5e-7
+    vs 6.5e-8).  Asserted: both forms inside the plain FP32 stated bound
+    2 gamma_{k+2}; BF16x3's sampled max within 2x and its mean within 4x of
+    the plain form's on the same entries."""
+    c = Case(n, n, n, kind="wide", seed=3 * n + 1, alpha=1.0, beta=0.0)
+    rng = np.random.default_rng(n)
+    ii = rng.integers(0, n, 8192).astype(np.int32)
+    jj = rng.integers(0, n, 8192).astype(np.int32)
+    ref, scale = orc.dgemm_entries(n, 1.0, c.A, n, c.B, n, 0.0, None, n, ii, jj)
+    dA, dB = upload(c.A), upload(c.B)
+    err = {}
+    for form, fn in ((sa.FORM_F32, sa.sgemm_ex), (sa.FORM_BF16X3, sa.sgemm_bf16x3)):
+        dC = upload(np.zeros(n * n, np.float32))
+        log(f"error comparison {n}^3 form {form}")
+        fn(n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n)
+        got = download(dC)[ii + jj.astype(np.int64) * n].astype(np.float64)
+        del dC
+        e = np.abs(got - ref) / scale
+        assert np.all(np.isfinite(e)) and np.all(e <= orc.fp32_tol_mult(n)), (form, e.max())
+        err[form] = (float(e.max()), float(e.mean()))
+    (fmax, fmean), (xmax, xmean) = err[sa.FORM_F32], err[sa.FORM_BF16X3]
+    log(f"{n}^3 wide, 8192 sampled entries: plain max {fmax:.3e} mean {fmean:.3e} | "
+        f"bf16x3 max {xmax:.3e} mean {xmean:.3e} | ratio max {xmax / fmax:.2f} "
+        f"mean {xmean / fmean:.2f}")
+    assert xmax <= 2.0 * fmax and xmean <= 4.0 * fmean, err
+
+
 def test_error_within_fp32_bound_at_8192():
     """At k = 8192 the BF16x3 result must meet the plain FP32 stated bound
     2 gamma_{k+2} scale (sampled entries against FP64)."""

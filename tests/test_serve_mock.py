@@ -196,3 +196,61 @@ def test_small_requests_leave_room_for_a_waiting_large_one():
         _, q = srv.latency(rid)
         assert q >= q_large - 0.5
     srv.destroy()
+
+
+def test_panel_failure_is_reported_after_inflight_panels():
+    """One panel of a large request fails at launch while another panel of
+    it is still running on the other slot: the request's error is published
+    only once that panel has retired (the caller may free the operands on
+    the error), and its remaining panels are never launched."""
+    srv = sa.Server(256 * MB, max_batch=8, num_slots=2, executor=sa.EXECUTOR_MOCK,
+                    mock_tflops=0.01, mock_launch_us=0, chunk_gflop=0.01, mock_fail_launch=2)
+    t0 = time.perf_counter()
+    large = _submit(srv, 512, 1024, 512)   # 4 panels of ~13.4 ms; launch 2 fails
+    time.sleep(0.003)
+    assert srv.poll(large) == 6            # panel 0 still in flight: not reported yet
+    assert srv.wait(large, 10) == 3        # SGEMM_STATUS_HIP_ERROR
+    elapsed_ms = (time.perf_counter() - t0) * 1e3
+    assert elapsed_ms >= 10.0, elapsed_ms  # waited for panel 0 (13.4 ms of mock compute)
+    st = srv.stats()
+    assert st["failed"] == 1 and st["completed"] == 0
+    assert st["panel_launches"] == 2       # panels 2 and 3 were dropped
+    # the server keeps serving afterwards
+    ok = _submit(srv, 64, 64, 64)
+    assert srv.wait(ok, 10) == 0
+    with pytest.raises(sa.SgemmError):     # drain reports that a request failed
+        srv.drain(10)
+    srv.destroy()
+
+
+def test_batch_failure_fails_only_its_requests():
+    srv = sa.Server(64 * MB, max_batch=2, num_slots=1, executor=sa.EXECUTOR_MOCK,
+                    mock_tflops=0.001, mock_launch_us=0, mock_fail_launch=1)
+    ids = [_submit(srv, 64, 64, 64) for _ in range(6)]
+    for rid in ids:
+        srv.wait(rid, 10)
+    codes = [srv.poll(rid) for rid in ids]
+    assert codes.count(3) >= 1 and codes.count(3) <= 2, codes   # the first batch only
+    assert codes.count(0) == len(ids) - codes.count(3), codes
+    assert codes[-1] == 0
+    st = srv.stats()
+    assert st["failed"] == codes.count(3) and st["measured"] == codes.count(0)
+    srv.destroy()
+
+
+def test_history_is_bounded_and_stats_stay_complete():
+    """Finished requests leave the history past 65536 (their ids return
+    NOT_FOUND); the running statistics still cover every request."""
+    srv = sa.Server(1 << 30, max_batch=256, num_slots=4, executor=sa.EXECUTOR_MOCK,
+                    mock_tflops=1e6, mock_launch_us=0)
+    count = 65536 + 1000
+    ids = [_submit(srv, 8, 8, 8) for _ in range(count)]
+    srv.drain(120)
+    codes = [srv.poll(rid) for rid in ids]
+    assert codes.count(7) == 1000 and codes.count(0) == 65536    # oldest 1000 evicted
+    assert codes[0] == 7 and codes[-1] == 0
+    st = srv.stats()
+    assert st["completed"] == count and st["measured"] == count
+    srv.reset_stats()
+    assert srv.stats()["measured"] == 0
+    srv.destroy()

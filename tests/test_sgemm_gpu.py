@@ -84,12 +84,19 @@ def test_config1_1024(alpha, beta):
     assert_ok(st, f"1024^3 alpha={alpha} beta={beta}")
 
 
-@pytest.mark.parametrize("n", [3072, 5120, 7168, 9216, 12288])
-def test_square_sweep_sampled_fp64(n):
-    """More squares of the 1024..16384 sweep: sampled entries against FP64."""
+SWEEP_SQUARES = [3072, 4096, 5120, 6144, 7168, 9216, 10240, 11264, 12288, 13312, 14336, 15360]
+
+
+@pytest.mark.parametrize("n", SWEEP_SQUARES)
+@pytest.mark.parametrize("form", [sa.FORM_F32, sa.FORM_BF16X3])
+def test_square_sweep_sampled_fp64(form, n):
+    """The squares of the 1024..16384 step-1024 sweep not covered below
+    (1024, 2048, 8192 and 16384 are), in both forms, with the configs the
+    default entry points pick: sampled entries against FP64."""
     c = Case(n, n, n, seed=n)
-    out = run_form(c, sa.FORM_F32)
-    _sampled_check(c, out, n_samples=2048, seed=n)
+    out = run_form(c, form)
+    tol = orc.fp32_tol_mult(n) if form == sa.FORM_F32 else orc.bf16x3_tol_mult(n)
+    _sampled_check(c, out, n_samples=2048, seed=n, tol_mult=tol)
 
 
 @pytest.mark.parametrize("config", range(N_F32_CONFIGS))
@@ -224,7 +231,7 @@ def test_stream_argument():
 
 
 # ---------------------------------------------------------------- full sizes
-def _sampled_check(c: Case, out: np.ndarray, n_samples=4096, seed=0):
+def _sampled_check(c: Case, out: np.ndarray, n_samples=4096, seed=0, tol_mult=None):
     rng = np.random.default_rng(seed)
     ii = rng.integers(0, c.m, n_samples).astype(np.int32)
     jj = rng.integers(0, c.n, n_samples).astype(np.int32)
@@ -233,7 +240,7 @@ def _sampled_check(c: Case, out: np.ndarray, n_samples=4096, seed=0):
     jj = np.concatenate([jj, [0, 0, c.n - 1, c.n - 1]]).astype(np.int32)
     ref, scale = orc.dgemm_entries(c.k, c.alpha, c.A, c.lda, c.B, c.ldb, c.beta, c.C0, c.ldc, ii, jj)
     got = out[ii + jj.astype(np.int64) * c.ldc].astype(np.float64)
-    tol = orc.fp32_tol_mult(c.k) * scale
+    tol = (orc.fp32_tol_mult(c.k) if tol_mult is None else tol_mult) * scale
     err = np.abs(got - ref)
     assert np.all(np.isfinite(got)) and np.all(err <= tol), \
         f"{(c.m, c.n, c.k)}: max err/tol {float(np.max(err / tol))}"
