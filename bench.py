@@ -9,10 +9,17 @@ uniform in [-1, 1) (synthetic, seeded; resident in HBM before timing starts).
 The flop convention is the reference's 2*M*N*K (benchmark.cu:138).
 
 Prints ONE JSON line.  `value` is the plain FP32 form's TFLOPS; the BF16x3
-form, its speedup and the error of both forms against an FP64 matmul of the
-same inputs ride along under "forms".  `roofline` prices the plain kernel
-against the FP32 matrix-core peak; `cpu_baseline` times the oracle's OpenMP
-FP32 restatement of the reference's naive kernel on a bounded sample.
+form, its speedup and the error of both forms on wide-range inputs ride
+along under "forms" (8192^3) and "forms_large" (16384^3): over the whole
+matrix against an FP64 matmul on the GPU, and on 8192 fixed entries against
+FP64 dot products computed on the host.  "error_target" says per size
+whether BF16x3 reaches 2x the plain TFLOPS at a max error no larger than the
+plain form's.  "vendor" times the vendor BLAS (hipBLASLt, rocBLAS) on the
+same problem, "sweep" both forms over the square sizes 1024..16384 and three
+odd shapes, "grouped" the 512-problem persistent launch, "serving" the
+request loop.  `roofline` prices the plain kernel against the FP32
+matrix-core peak; `cpu_baseline` times the oracle's OpenMP FP32 restatement
+of the reference's naive kernel on a bounded sample (and the FP64 oracle).
 """
 from __future__ import annotations
 
@@ -44,6 +51,10 @@ def parse():
     ap.add_argument("--large-size", type=int, default=16384,
                     help="also time both forms, and their FP64 error, at this size (0 = skip)")
     ap.add_argument("--no-grouped", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the compact size sweep")
+    ap.add_argument("--no-vendor", action="store_true", help="skip the vendor BLAS line")
+    ap.add_argument("--tune-iters", type=int, default=10,
+                    help="timed launches per candidate when the grouped leg tunes its buckets")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--group-count", type=int, default=512)
     ap.add_argument("--serve-seconds", type=float, default=2.0,
@@ -66,8 +77,10 @@ def parse():
     args = ap.parse_args()
     if args.plain_only:
         args.no_bf16x3 = args.no_error = args.no_cpu = args.no_grouped = args.no_serve = True
+        args.no_sweep = args.no_vendor = True
     if args.x3_only:
         args.no_error = args.no_cpu = args.no_grouped = args.no_serve = True
+        args.no_sweep = args.no_vendor = True
     return args
 
 
@@ -105,6 +118,75 @@ def sweep(sa, torch, warmup):
                       "peaks": {"f32": PEAK_FP32_TFLOPS, "bf16x3_useful": PEAK_BF16_TFLOPS / 3},
                       "alpha": 1.5, "beta": 0.5, "data": "synthetic uniform [-1,1)",
                       "rows": out}), flush=True)
+
+
+def sweep_compact(sa, torch, stream, target_s=0.08):
+    """The size sweep in brief (both forms, TFLOPS and fraction of peak per
+    size): each (size, form) is warmed up, then timed over about target_s
+    seconds of back-to-back calls with HIP events on the launch stream."""
+    shapes = [(s, s, s) for s in range(1024, 16385, 1024)]
+    shapes += [(1000, 1000, 1000), (4097, 3071, 2049), (8192, 8192, 127)]
+    rows = []
+    for (m, n, k) in shapes:
+        A = torch.rand(m * k, device="cuda") * 2 - 1
+        B = torch.rand(k * n, device="cuda") * 2 - 1
+        C = torch.rand(m * n, device="cuda") * 2 - 1
+        flops = 2.0 * m * n * k
+        row = {"m": m, "n": n, "k": k}
+        for form, name, fn, peak in ((sa.FORM_F32, "f32", sa.sgemm_ex, PEAK_FP32_TFLOPS),
+                                     (sa.FORM_BF16X3, "bf16x3", sa.sgemm_bf16x3,
+                                      PEAK_BF16_TFLOPS / 3)):
+            def call():
+                fn(m, n, k, 1.5, A, m, B, k, 0.5, C, m, stream)
+            probe = time_form(call, 2, 1, stream)  # ms per call, cold
+            it = max(3, min(400, int(target_s * 1e3 / max(probe, 1e-3))))
+            ms = time_form(call, it, max(2, it // 2), stream)
+            tf = flops / (ms * 1e-3) / 1e12
+            row[name] = {"tflops": tf, "frac_of_peak": tf / peak, "ms": ms, "calls": it,
+                         "config": sa.autotune_lookup(form, m, n, k)[0]}
+        rows.append(row)
+        del A, B, C
+    torch.cuda.empty_cache()
+    return {"alpha": 1.5, "beta": 0.5, "data": "synthetic uniform [-1,1)",
+            "peaks": {"f32": PEAK_FP32_TFLOPS, "bf16x3_useful": PEAK_BF16_TFLOPS / 3},
+            "rows": rows}
+
+
+def vendor_leg(torch, n, A, B, C0, steps, warmup, stream):
+    """The vendor BLAS on the same inputs and problem (reference:
+    benchmark.cu's CUBLAS=1 line): torch's FP32 addmm, which calls hipBLASLt
+    or rocBLAS on ROCm, with each library that torch lets us prefer.  The
+    column-major C = 1.5 A B + 0.5 C is the row-major C^T = 1.5 B^T A^T +
+    0.5 C^T, which is what the row-major views below compute."""
+    out = {}
+    Av = A.view(n, n)   # row-major view of column-major A: A^T
+    Bv = B.view(n, n)   # B^T
+    flops = 2.0 * n ** 3
+    prev = None
+    try:
+        prev = torch.backends.cuda.preferred_blas_library()
+    except Exception:
+        pass
+    torch.backends.cuda.matmul.allow_tf32 = False
+    for name, lib in (("hipblaslt", "cublaslt"), ("rocblas", "cublas")):
+        try:
+            got = torch.backends.cuda.preferred_blas_library(lib)
+        except Exception as e:  # noqa: BLE001 - report and go on
+            out[name] = {"error": str(e)}
+            continue
+        C = C0.clone()
+        Cv = C.view(n, n)
+        ms = time_form(lambda: Cv.addmm_(Bv, Av, beta=0.5, alpha=1.5), steps, warmup, stream)
+        out[name] = {"library_set": str(got), "ms": ms,
+                     "tflops": flops / (ms * 1e-3) / 1e12}
+        del C, Cv
+    if prev is not None:
+        try:
+            torch.backends.cuda.preferred_blas_library(prev)
+        except Exception:
+            pass
+    torch.cuda.empty_cache()
+    return out
 
 
 def copy_calibration(steps):
@@ -228,9 +310,29 @@ def error_vs_fp64(C, ref64, scale64):
             "max_rel": float(rel.max()), "mean_rel": float(rel.mean())}
 
 
-def wide_error(sa, torch, n, with_x3, stream):
-    """Wide-range inputs (sign * 2^u, u in [-8, 8)), FP64 torch matmul as
-    truth: the componentwise and relative error of each form at n^3."""
+def host_fp64_entries(A, B, n, ii, jj, chunk=512):
+    """FP64 reference and scale |A||B| of entries (ii, jj) of the column-major
+    n x n product of host float32 operands: float64 dot products (numpy), the
+    computation the FP64 host oracle restates for the tests."""
+    import numpy as np
+    Am = A.reshape(n, n)   # Am[p, i] = A(i, p)
+    Bm = B.reshape(n, n)   # Bm[j, p] = B(p, j)
+    ref = np.empty(len(ii))
+    scale = np.empty(len(ii))
+    for s in range(0, len(ii), chunk):
+        a = Am[:, ii[s:s + chunk]].T.astype(np.float64)
+        b = Bm[jj[s:s + chunk], :].astype(np.float64)
+        ref[s:s + chunk] = np.einsum("ij,ij->i", a, b)
+        scale[s:s + chunk] = np.einsum("ij,ij->i", np.abs(a), np.abs(b))
+    return ref, scale
+
+
+def wide_error(sa, torch, n, with_x3, stream, samples=8192):
+    """Wide-range inputs (sign * 2^u, u in [-8, 8)): the componentwise and
+    relative error of each form at n^3 over the whole matrix against an FP64
+    torch matmul on the GPU, and over `samples` fixed entries against FP64
+    dot products computed on the host from the same (downloaded) operands."""
+    import numpy as np
     dev = "cuda"
     g = torch.Generator(device=dev)
     g.manual_seed(1)
@@ -240,6 +342,11 @@ def wide_error(sa, torch, n, with_x3, stream):
         s = torch.where(torch.rand(n * n, device=dev, generator=g) < 0.5, -1.0, 1.0)
         return (s.double() * torch.exp2(u)).float()
     Aw, Bw = wide(), wide()
+    rng = np.random.default_rng(n)
+    ii = rng.integers(0, n, samples)
+    jj = rng.integers(0, n, samples)
+    href, hscale = host_fp64_entries(Aw.cpu().numpy(), Bw.cpu().numpy(), n, ii, jj)
+    idx = torch.from_numpy(ii + jj * n).to(dev)
     Am = Aw.view(n, n).t().double()   # column-major -> logical (m, k)
     Bm = Bw.view(n, n).t().double()
     ref = (Am @ Bm).t().contiguous().view(-1)
@@ -247,11 +354,15 @@ def wide_error(sa, torch, n, with_x3, stream):
     del Am, Bm
     Cw = torch.zeros(n * n, device=dev)
     out = {}
-    sa.sgemm_ex(n, n, n, 1.0, Aw, n, Bw, n, 0.0, Cw, n, stream)
-    out["f32"] = error_vs_fp64(Cw, ref, scale)
-    if with_x3:
-        sa.sgemm_bf16x3(n, n, n, 1.0, Aw, n, Bw, n, 0.0, Cw, n, stream)
-        out["bf16x3"] = error_vs_fp64(Cw, ref, scale)
+    forms = [("f32", sa.sgemm_ex)] + ([("bf16x3", sa.sgemm_bf16x3)] if with_x3 else [])
+    for name, fn in forms:
+        fn(n, n, n, 1.0, Aw, n, Bw, n, 0.0, Cw, n, stream)
+        e = error_vs_fp64(Cw, ref, scale)
+        got = Cw[idx].double().cpu().numpy()
+        comp = np.abs(got - href) / hscale
+        e["sampled_host_fp64"] = {"samples": samples, "max_comp": float(comp.max()),
+                                  "mean_comp": float(comp.mean())}
+        out[name] = e
     del ref, scale, Aw, Bw, Cw
     torch.cuda.empty_cache()
     return out
@@ -305,6 +416,11 @@ def cpu_baseline(size: int, cols: int):
         orc.sgemm_basic_cm(s, s, s, 1.5, A1, s, B1, s, 0.5, C1, s)
         times.append(time.perf_counter() - t1)
     t1024 = float(np.median(times))
+    # the FP64 host oracle (the ground truth of the tests) on a bounded sample
+    cols64 = min(256, cols)
+    t2 = time.perf_counter()
+    orc.dgemm(m, cols64, k, 1.5, A, m, B, k, 0.5, C, m, with_scale=False)
+    dt64 = time.perf_counter() - t2
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -319,10 +435,12 @@ def cpu_baseline(size: int, cols: int):
             "sample": f"{cols} of {size} output columns of the {size}^3 problem "
                       f"(M={m}, N={cols}, K={k}), alpha=1.5 beta=0.5, {dt:.1f} s",
             "tflops_1024": 2.0 * s ** 3 / t1024 / 1e12, "ms_1024_median_of_5": t1024 * 1e3,
+            "fp64_oracle": {"tflops": 2.0 * m * cols64 * k / dt64 / 1e12, "seconds": dt64,
+                            "sample": f"M={m}, N={cols64}, K={k}, alpha=1.5 beta=0.5"},
             "host_cpus_visible": os.cpu_count(), "cpu_model": model}
 
 
-def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0):
+def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0, tune_iters=10):
     """512 independent problems, M, N, K log-uniform integers in [32, 2048],
     U[-1, 1), alpha=1, beta=0: one persistent launch replayed from a hipGraph,
     against the same problems as back-to-back single launches of the default
@@ -347,7 +465,8 @@ def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0):
     for m, n, k in shapes.tolist():
         key = tuple(int(v - 1).bit_length() for v in (m, n, k))
         if key not in tuned:
-            tuned[key] = sa.autotune_bucket(form, m, n, k, iters=2, grouped=True, stream=stream)[0]
+            tuned[key] = sa.autotune_bucket(form, m, n, k, iters=tune_iters, grouped=True,
+                                            stream=stream)[0]
     tune_s = time.perf_counter() - t0
     torch.cuda.synchronize()
     gs = torch.cuda.Stream()  # stream capture needs a stream other than the null stream
@@ -373,7 +492,7 @@ def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0):
     info = plan.info()
     plan.destroy()
     return {"problems": count, "flops": flops, "buckets_tuned": len(tuned),
-            "tune_s": tune_s, "ms_grouped": ms_group,
+            "tune_s": tune_s, "tune_iters": tune_iters, "ms_grouped": ms_group,
             "tflops_grouped": flops / (ms_group * 1e-3) / 1e12,
             "ms_single_loop": ms_loop, "tflops_single_loop": flops / (ms_loop * 1e-3) / 1e12,
             "speedup_vs_loop": ms_loop / ms_group, "tiles": info["total_tiles"],
@@ -478,6 +597,26 @@ def serving_leg(sa, torch, rates, window_s, chunk_gflop=0.0, seed=0):
     return out
 
 
+def error_target(forms, large):
+    """The split-precision target, per size: BF16x3 at least 2x the plain
+    form's TFLOPS with a max componentwise error no larger than the plain
+    form's, on the whole matrix (FP64 torch matmul) and on the sampled
+    entries (host FP64)."""
+    out = {}
+    for label, f in (("n8192", forms), ("n16384", large)):
+        if not f or "bf16x3" not in f or "error_wide" not in f.get("bf16x3", {}):
+            continue
+        e32, e3 = f["f32"]["error_wide"], f["bf16x3"]["error_wide"]
+        speed = f["bf16x3"]["tflops"] / f["f32"]["tflops"]
+        full = e3["max_comp"] <= e32["max_comp"]
+        samp = (e3["sampled_host_fp64"]["max_comp"] <=
+                e32["sampled_host_fp64"]["max_comp"])
+        out[label] = {"speedup": speed, "max_full_matrix_ok": full, "max_sampled_ok": samp,
+                      "mean_ratio_full_matrix": e3["mean_comp"] / e32["mean_comp"],
+                      "met": bool(speed >= 2.0 and full and samp)}
+    return out
+
+
 def load_traffic(size: int):
     path = os.path.join(ROOT, "profiles", f"traffic_sgemm_f32_{size}.json")
     if not os.path.exists(path):
@@ -549,6 +688,14 @@ def main():
                            "per_call_flushed": per_call(split, min(args.steps, 10), stream,
                                                         flush)}
 
+    vendor = None
+    if not args.no_vendor:
+        print("bench: vendor BLAS", file=sys.stderr, flush=True)
+        vendor = vendor_leg(torch, n, A, B, C0, args.steps, args.warmup, stream)
+        for v in vendor.values():
+            if "tflops" in v:
+                v["ours_over_vendor"] = tflops / v["tflops"]
+
     if not args.no_error:
         print("bench: error vs FP64", file=sys.stderr, flush=True)
         errs = wide_error(sa, torch, n, "bf16x3" in forms, stream)
@@ -561,13 +708,18 @@ def main():
         large = forms_at(sa, torch, args.large_size, min(args.steps, 5), min(args.warmup, 2),
                          stream, not args.no_error)
 
+    sweep_rows = None
+    if not args.no_sweep:
+        print("bench: size sweep", file=sys.stderr, flush=True)
+        sweep_rows = sweep_compact(sa, torch, stream)
+
     grouped = None
     if not args.no_grouped:
         grouped = {}
         for form, name in ((sa.FORM_F32, "f32"), (sa.FORM_BF16X3, "bf16x3")):
             print(f"bench: grouped {name}", file=sys.stderr, flush=True)
             grouped[name] = grouped_leg(sa, torch, form, args.group_count, args.steps,
-                                        args.warmup, stream)
+                                        args.warmup, stream, tune_iters=args.tune_iters)
 
     serving = None
     if not args.no_serve:
@@ -602,6 +754,9 @@ def main():
                                f"beta=0.5", "m": n, "n": n, "k": n},
         "forms": forms,
         "forms_large": large,
+        "error_target": error_target(forms, large),
+        "vendor": vendor,
+        "sweep": sweep_rows,
         "grouped": grouped,
         "serving": serving,
         "roofline": roofline,
