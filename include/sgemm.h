@@ -120,6 +120,27 @@ int sgemm_bf16x3(int m, int n, int k, const float* alpha, const float* A, int ld
                  const float* B, int ldb, const float* beta, float* C, int ldc,
                  sgemm_stream_t stream);
 
+/* No-allocation form of sgemm_ex (form SGEMM_FORM_F32) and sgemm_bf16x3
+ * (form SGEMM_FORM_BF16X3).  No counterpart in the reference (its sgemm keeps
+ * no device state); for callers that cannot allocate device memory: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Same arguments, config choice, kernels and result bits as those
+ * entry points, but the split-K, tail-split and stream-K partials and arrival
+ * counters come from one static device array per GPU, held until the process
+ * ends (SGEMM_NOALLOC_SCRATCH_PARTIAL_BYTES of partials and
+ * SGEMM_NOALLOC_SCRATCH_COUNTERS counters), instead of a per-stream
+ * workspace: nothing is allocated, also under capture (where sgemm_ex would
+ * run such a problem unsplit).  A problem whose split plan needs more than
+ * the array holds returns SGEMM_STATUS_SCRATCH_TOO_SMALL and launches
+ * nothing.  Every call on one GPU shares that GPU's array: calls that overlap
+ * in time on one GPU -- on other streams, or replays of graphs that captured
+ * a call -- race on it and must be ordered by the caller. */
+#define SGEMM_NOALLOC_SCRATCH_PARTIAL_BYTES (64u << 20)
+#define SGEMM_NOALLOC_SCRATCH_COUNTERS 16384
+int sgemm_ex_noalloc(int form, int m, int n, int k, const float* alpha, const float* A, int lda,
+                     const float* B, int ldb, const float* beta, float* C, int ldc,
+                     sgemm_stream_t stream);
+
 /* Frees the split-K / stream-K workspace the library holds for `stream`
  * (waiting for the stream first).  Call it before destroying a stream that
  * ran split configs, so the handle cannot be reused while its workspace is

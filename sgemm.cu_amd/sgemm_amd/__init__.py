@@ -137,6 +137,8 @@ def lib() -> ctypes.CDLL:
     L.sgemm_ex.restype = i
     L.sgemm_bf16x3.argtypes = mm + [vp]
     L.sgemm_bf16x3.restype = i
+    L.sgemm_ex_noalloc.argtypes = [i] + mm + [vp]
+    L.sgemm_ex_noalloc.restype = i
     L.sgemm_run_config.argtypes = [i, i] + mm + [vp]
     L.sgemm_run_config.restype = i
     L.sgemm_config_count.argtypes = [i]
@@ -256,6 +258,16 @@ def sgemm_bf16x3(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream=None, chec
                             ldc, _stream(stream))
     if check:
         _check(st, "sgemm_bf16x3")
+    return st
+
+
+def sgemm_ex_noalloc(form, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream=None,
+                     check=True) -> int:
+    """sgemm_ex / sgemm_bf16x3 with scratch from a static device array (no allocation)."""
+    st = lib().sgemm_ex_noalloc(form, m, n, k, _f(alpha), _ptr(A), lda, _ptr(B), ldb, _f(beta),
+                                _ptr(C), ldc, _stream(stream))
+    if check:
+        _check(st, "sgemm_ex_noalloc")
     return st
 
 
