@@ -26,17 +26,29 @@ int record(int status) {
     return status;
 }
 
+}  // namespace
+
 // Column-major BLAS NN argument rules.  A is m x k, B is k x n, C is m x n.
-int validate(int m, int n, int k, const float* alpha, const void* A, int lda, const void* B,
-             int ldb, const float* beta, const void* C, int ldc) {
+int sgemm_amd::validate_args(int m, int n, int k, const float* alpha, const void* A, int lda,
+                             const void* B, int ldb, const float* beta, const void* C,
+                             int ldc) {
     if (m < 0 || n < 0 || k < 0) return SGEMM_STATUS_INVALID_VALUE;
     if (alpha == nullptr || beta == nullptr) return SGEMM_STATUS_INVALID_VALUE;
     if (lda < (m > 1 ? m : 1) || ldb < (k > 1 ? k : 1) || ldc < (m > 1 ? m : 1))
         return SGEMM_STATUS_INVALID_VALUE;
     if (m == 0 || n == 0) return SGEMM_STATUS_SUCCESS;
-    if (C == nullptr) return SGEMM_STATUS_INVALID_VALUE;
-    if (k > 0 && *alpha != 0.f && (A == nullptr || B == nullptr)) return SGEMM_STATUS_INVALID_VALUE;
+    if (C == nullptr || !dword_aligned(C)) return SGEMM_STATUS_INVALID_VALUE;
+    if (k > 0 && *alpha != 0.f &&
+        (A == nullptr || B == nullptr || !dword_aligned(A) || !dword_aligned(B)))
+        return SGEMM_STATUS_INVALID_VALUE;
     return SGEMM_STATUS_SUCCESS;
+}
+
+namespace {
+
+int validate(int m, int n, int k, const float* alpha, const void* A, int lda, const void* B,
+             int ldb, const float* beta, const void* C, int ldc) {
+    return validate_args(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 Problem make_problem(int m, int n, int k, float alpha, const float* A, int lda, const float* B,

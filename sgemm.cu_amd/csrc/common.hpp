@@ -26,11 +26,9 @@ struct Problem {
     int tiles_m, tiles_n;  // tile grid of this problem for the chosen config
     int tile_begin;        // first global tile index (grouped schedules)
     int cfg;               // config id the tiles were planned with
-    int flags;             // kProblemVec4: every 16-byte vector access is aligned
+    int flags;             // reserved, zero
     int reserved;
 };
-
-constexpr int kProblemVec4 = 1;
 
 // Split-K state of one workgroup: it computes k in [split*kchunk, ...) of
 // tile `tile`, dumps its accumulators to `partials`, and the last of the
@@ -121,20 +119,108 @@ __device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base, long long b
                                              0x00020000);
 }
 
-// Four consecutive floats at byte offset `off` of a buffer.  VEC: one 16-byte
-// load (offset 16-byte aligned); otherwise four 4-byte loads.
-template <bool VEC>
+// Four consecutive floats at byte offset `off` of a buffer: one 16-byte load.
+// Floats are 4-byte aligned and a multi-dword access needs only dword
+// alignment, so this serves every leading dimension; the range check is per
+// dword, so a load that reaches past the end of the buffer still returns the
+// floats below it (and 0 beyond).
 __device__ inline f32x4 bload4(__amdgpu_buffer_rsrc_t r, int off) {
-    if constexpr (VEC) {
-        return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
-    } else {
-        f32x4 v;
-        v[0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-        v[1] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off + 4, 0, 0));
-        v[2] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off + 8, 0, 0));
-        v[3] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off + 12, 0, 0));
+    return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+
+// Rows r..r+3 of one column, zero outside [0, valid): one 16-byte load when
+// all four are valid (dword alignment suffices, see bload4).
+__device__ inline f32x4 load4(const float* p, int valid) {
+    f32x4 v = {0.f, 0.f, 0.f, 0.f};
+    if (valid >= 4) {
+        __builtin_memcpy(&v, p, sizeof(v));
         return v;
     }
+    if (valid > 0) v[0] = p[0];
+    if (valid > 1) v[1] = p[1];
+    if (valid > 2) v[2] = p[2];
+    return v;
+}
+
+// The first valid (<= 4) of rows r..r+3 of one column.
+__device__ inline void store4(float* p, f32x4 v, int valid) {
+    if (valid >= 4) {
+        __builtin_memcpy(p, &v, sizeof(v));
+        return;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+        if (r < valid) p[r] = v[r];
+}
+
+// Floats at the start of a tile's LDS that the epilogue leaves alone: the
+// split-K arrival flag lives there while the other waves still read it.
+constexpr int kEpilogueLdsSkip = 16;
+
+// LDS floats the epilogue of a tile of `waves` waves with TM-row wave tiles
+// and MF-column MFMA blocks needs (kEpilogueLdsSkip included).
+constexpr int epilogue_lds_floats(int waves, int TM, int MF) {
+    return kEpilogueLdsSkip + waves * MF * (TM + 4);
+}
+
+// Epilogue of one MF-column block of a wave's TM-row tile, through LDS.  The
+// MFMA output layout gives a lane four consecutive rows of one column per
+// register group, so a store straight from it touches MF columns with 16 to
+// 64 bytes each -- part of a cache line per column.  Here the block goes to a
+// wave-private LDS area [column][row] (row stride TM + 4 floats) and back so
+// that a lane holds four consecutive rows of a column and the wave's 64
+// lanes cover 256 / TM whole columns of the tile: every load and store of C
+// is a run of whole 16-byte pieces down a column.  The arithmetic per
+// element is unchanged, fma(beta, c, alpha * acc) or alpha * acc with beta ==
+// 0 (C not read).
+//   wlds:  MF * (TM + 4) floats of LDS used by this wave only; the LDS must
+//          be free (every wave of the workgroup past its last use of it).
+//   m0w:   the wave tile's first row; col0: the block's first column.
+//   vals(i, q): the block's MFMA block i, register group q (4 rows).
+//   c_at(p, cc, valid): the C values of read-back pass p at cc (beta != 0).
+// DS instructions of one wave complete in order, so the read-back sees the
+// writes without a barrier.
+template <int TM, int MF, int MB, class Vals, class CAt>
+__device__ inline void epilogue_block(float* wlds, const Problem& P, int m0w, int col0,
+                                      Vals&& vals, CAt&& c_at) {
+    constexpr int TMP = TM + 4;
+    constexpr int GROUPS = MF == 16 ? 1 : 4;
+    constexpr int LPC = TM / 4;   // lanes per column on read-back
+    constexpr int CPP = 64 / LPC; // columns per read-back pass
+    static_assert(MF == 16 || MF == 32, "MFMA block");
+    static_assert(64 % LPC == 0 && MF % CPP == 0, "read-back mapping");
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int q = 0; q < GROUPS; ++q) {
+            const int row = MF * i + (MF == 16 ? 4 * (lane / 16) : 8 * q + 4 * (lane / 32));
+            *reinterpret_cast<f32x4*>(&wlds[(lane % MF) * TMP + row]) = vals(i, q);
+        }
+    __builtin_amdgcn_wave_barrier();
+    const int c_in = lane / LPC, r = 4 * (lane % LPC);
+    const int row = m0w + r;
+    const int valid = min(max(P.m - row, 0), 4);
+    const float alpha = P.alpha, beta = P.beta;
+#pragma unroll
+    for (int p = 0; p < MF / CPP; ++p) {
+        const int c = p * CPP + c_in;
+        const f32x4 a = *reinterpret_cast<const f32x4*>(&wlds[c * TMP + r]);
+        const int col = col0 + c;
+        if (col >= P.n || valid == 0) continue;
+        float* cc = P.C + (size_t)col * P.ldc + row;
+        f32x4 v;
+        if (beta != 0.f) {
+            const f32x4 c0 = c_at(p, cc, valid);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = __builtin_fmaf(beta, c0[e], alpha * a[e]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = alpha * a[e];
+        }
+        store4(cc, v, valid);
+    }
+    __builtin_amdgcn_wave_barrier();
 }
 
 // Number of XCDs the dispatcher round-robins workgroups over (MI355X: 8).

@@ -70,9 +70,9 @@ static_assert(X3_64x64::LDS_BYTES <= kGroupedLdsX3, "X3 grouped LDS");
 constexpr int kGroupedWpeF32 = 3;
 constexpr int kGroupedWpeX3 = 2;
 
-template <class T, bool VEC>
+template <class T>
 __device__ inline void run_tile_f32(const Problem& P, int tm, int tn, char* smem) {
-    F32TileRunner<T, VEC, false>::run(P, tm, tn, reinterpret_cast<float*>(smem));
+    F32TileRunner<T, false>::run(P, tm, tn, reinterpret_cast<float*>(smem));
 }
 
 template <int FORM>
@@ -109,22 +109,16 @@ void k_grouped(GroupHeader* __restrict__ hdr, const Problem* __restrict__ probs)
         const Problem P = probs[lo];
         int tm, tn;
         tile_coords(w - P.tile_begin, P.tiles_m, P.tiles_n, 8, tm, tn);
-        const bool vec = (P.flags & kProblemVec4) != 0;
         if constexpr (FORM == kFormF32) {
             switch (P.cfg) {
-                case 5: vec ? run_tile_f32<F32_128x128_k16, true>(P, tm, tn, smem)
-                            : run_tile_f32<F32_128x128_k16, false>(P, tm, tn, smem); break;
-                case 1: vec ? run_tile_f32<F32_64x64, true>(P, tm, tn, smem)
-                            : run_tile_f32<F32_64x64, false>(P, tm, tn, smem); break;
-                default: vec ? run_tile_f32<F32_128x64, true>(P, tm, tn, smem)
-                             : run_tile_f32<F32_128x64, false>(P, tm, tn, smem); break;
+                case 5: run_tile_f32<F32_128x128_k16>(P, tm, tn, smem); break;
+                case 1: run_tile_f32<F32_64x64>(P, tm, tn, smem); break;
+                default: run_tile_f32<F32_128x64>(P, tm, tn, smem); break;
             }
         } else {
             switch (P.cfg) {
-                case 8: vec ? X3TileRunner<X3_128x128_2cu, true, false>::run(P, tm, tn, smem)
-                            : X3TileRunner<X3_128x128_2cu, false, false>::run(P, tm, tn, smem); break;
-                default: vec ? X3TileRunner<X3_64x64, true, false>::run(P, tm, tn, smem)
-                             : X3TileRunner<X3_64x64, false, false>::run(P, tm, tn, smem); break;
+                case 8: X3TileRunner<X3_128x128_2cu, false>::run(P, tm, tn, smem); break;
+                default: X3TileRunner<X3_64x64, false>::run(P, tm, tn, smem); break;
             }
         }
         __syncthreads();  // the tile is done with the LDS
@@ -178,8 +172,9 @@ int validate_problem(const sgemm_problem& q) {
         return SGEMM_STATUS_INVALID_VALUE;
     if (q.m > 0 && q.n > 0) {
         if (q.lda > kMaxLd || q.ldb > kMaxLd) return SGEMM_STATUS_NOT_SUPPORTED;
-        if (q.C == nullptr) return SGEMM_STATUS_INVALID_VALUE;
-        if (q.k > 0 && q.alpha != 0.f && (q.A == nullptr || q.B == nullptr))
+        if (q.C == nullptr || !dword_aligned(q.C)) return SGEMM_STATUS_INVALID_VALUE;
+        if (q.k > 0 && q.alpha != 0.f &&
+            (q.A == nullptr || q.B == nullptr || !dword_aligned(q.A) || !dword_aligned(q.B)))
             return SGEMM_STATUS_INVALID_VALUE;
     }
     return SGEMM_STATUS_SUCCESS;
@@ -200,7 +195,6 @@ Problem to_problem(const sgemm_problem& q) {
     p.ldc = q.ldc;
     p.alpha = q.alpha;
     p.beta = q.beta;
-    p.flags = vec4_ok(p) ? kProblemVec4 : 0;
     return p;
 }
 

@@ -146,6 +146,11 @@ hipError_t launch_wide(const Problem& p, hipStream_t stream);
 // C := beta*C for the k == 0 / alpha == 0 shortcut (sgemm_f32.hip).
 hipError_t launch_scale_c(const Problem& p, hipStream_t stream);
 
+// The column-major argument rules of the single-problem entry points
+// (api.cpp): SGEMM_STATUS_SUCCESS or SGEMM_STATUS_INVALID_VALUE.
+int validate_args(int m, int n, int k, const float* alpha, const void* A, int lda,
+                  const void* B, int ldb, const float* beta, const void* C, int ldc);
+
 // The shared body of the single-problem entry points (api.cpp): validation,
 // the k == 0 / alpha == 0 shortcut, the wide-ld fallback and the config
 // launch; config_id < 0 picks from the autotune table.  Returns an
@@ -162,11 +167,9 @@ inline const ConfigInfo& config_info(int form, int id) {
     return form == kFormF32 ? f32_config(id) : bf16x3_config(id);
 }
 
-// True when every float4 access of the vector path is 16-byte aligned.
-inline bool vec4_ok(const Problem& p) {
-    auto al = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
-    return al(p.A) && al(p.B) && al(p.C) && (p.lda % 4 == 0) && (p.ldb % 4 == 0) &&
-           (p.ldc % 4 == 0);
-}
+// Operands must be 4-byte aligned (every float pointer is): the tiles load
+// and store four floats at a time with 16-byte accesses, which need dword
+// alignment only, whatever the leading dimensions.
+inline bool dword_aligned(const void* q) { return (reinterpret_cast<uintptr_t>(q) & 3u) == 0; }
 
 }  // namespace sgemm_amd

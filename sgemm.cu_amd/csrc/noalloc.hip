@@ -99,7 +99,10 @@ extern "C" int sgemm_ex_noalloc(int form, int m, int n, int k, const float* alph
                                 const float* A, int lda, const float* B, int ldb,
                                 const float* beta, float* C, int ldc, sgemm_stream_t stream) {
     using namespace sgemm_amd;
+    const int st = validate_args(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (st != SGEMM_STATUS_SUCCESS) return st;
     if (form != SGEMM_FORM_F32 && form != SGEMM_FORM_BF16X3) return SGEMM_STATUS_NOT_SUPPORTED;
+    if (m == 0 || n == 0) return SGEMM_STATUS_SUCCESS;
     hipStream_t s = static_cast<hipStream_t>(stream);
     Arena arena{};
     const hipError_t e = device_arena(s, &arena);

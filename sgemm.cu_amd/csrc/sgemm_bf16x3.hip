@@ -10,7 +10,7 @@ constexpr int kX3GroupN = 8;
 // SPLIT: workgroups [0, sk.dp_tiles) compute tiles [0, sk.dp_tiles) whole;
 // the rest are (tile, split) units of the remaining tiles, split index
 // outermost (as sgemm_f32.hip).
-template <class T, bool VEC, bool SPLIT>
+template <class T, bool SPLIT>
 __global__ __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(T::WPE)))
 void k_sgemm_bf16x3(Problem P, SplitK sk) {
     __shared__ __attribute__((aligned(16))) char smem[T::LDS_BYTES];
@@ -19,7 +19,7 @@ void k_sgemm_bf16x3(Problem P, SplitK sk) {
     if (!SPLIT || b < sk.dp_tiles) {
         tile_coords(xcd_remap(b, SPLIT ? sk.dp_tiles : (int)gridDim.x), P.tiles_m, P.tiles_n,
                     kX3GroupN, tm, tn);
-        X3TileRunner<T, VEC>::run(P, tm, tn, smem);
+        X3TileRunner<T>::run(P, tm, tn, smem);
     } else {
         const int units = (int)gridDim.x - sk.dp_tiles;
         const int u = xcd_remap(b - sk.dp_tiles, units);
@@ -27,7 +27,7 @@ void k_sgemm_bf16x3(Problem P, SplitK sk) {
         const int split = u / count;
         const int slot = u - split * count;
         tile_coords(sk.dp_tiles + slot, P.tiles_m, P.tiles_n, kX3GroupN, tm, tn);
-        X3TileRunner<T, VEC>::run(P, tm, tn, smem, &sk, split, slot);
+        X3TileRunner<T>::run(P, tm, tn, smem, &sk, split, slot);
     }
 }
 
@@ -38,7 +38,7 @@ void k_sgemm_bf16x3(Problem P, SplitK sk) {
 // tile wholly inside one workgroup's range is stored straight to C; a tile
 // cut at a workgroup boundary is reduced from its pieces in k order by the
 // last of them to finish.
-template <class T, bool VEC>
+template <class T>
 __global__ __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(T::WPE)))
 void k_sgemm_bf16x3_sk(Problem P, SplitK sk) {
     __shared__ __attribute__((aligned(16))) char smem[T::LDS_BYTES];
@@ -46,12 +46,12 @@ void k_sgemm_bf16x3_sk(Problem P, SplitK sk) {
     const int b = blockIdx.x;
     if (b < sk.dp_tiles) {
         tile_coords(xcd_remap(b, sk.dp_tiles), P.tiles_m, P.tiles_n, kX3GroupN, tm, tn);
-        X3TileRunner<T, VEC>::run(P, tm, tn, smem);
+        X3TileRunner<T>::run(P, tm, tn, smem);
         return;
     }
     streamk_pieces(P, sk, b, T::BK, [&](int tile, const KPiece& pc) {
         tile_coords(tile, P.tiles_m, P.tiles_n, kX3GroupN, tm, tn);
-        X3TileRunner<T, VEC>::run_piece(P, tm, tn, smem, &sk, pc);
+        X3TileRunner<T>::run_piece(P, tm, tn, smem, &sk, pc);
     });
 }
 
@@ -99,13 +99,8 @@ static hipError_t launch_x3(const Problem& p, hipStream_t s, const Arena* arena)
     constexpr bool S = ALLOW_SPLIT;
     const dim3 grid((unsigned)(tiles * sk.splits));
     (void)hipGetLastError();
-    if (vec4_ok(p)) {
-        if (split) hipLaunchKernelGGL((k_sgemm_bf16x3<T, true, S>), grid, dim3(T::NT), 0, s, q, sk);
-        else hipLaunchKernelGGL((k_sgemm_bf16x3<T, true, false>), grid, dim3(T::NT), 0, s, q, sk);
-    } else {
-        if (split) hipLaunchKernelGGL((k_sgemm_bf16x3<T, false, S>), grid, dim3(T::NT), 0, s, q, sk);
-        else hipLaunchKernelGGL((k_sgemm_bf16x3<T, false, false>), grid, dim3(T::NT), 0, s, q, sk);
-    }
+    if (split) hipLaunchKernelGGL((k_sgemm_bf16x3<T, S>), grid, dim3(T::NT), 0, s, q, sk);
+    else hipLaunchKernelGGL((k_sgemm_bf16x3<T, false>), grid, dim3(T::NT), 0, s, q, sk);
     return hipGetLastError();
 }
 
@@ -114,7 +109,7 @@ template <class T>
 static long long x3_resident_workgroups() {
     static const long long n = [] {
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sgemm_bf16x3<T, true, true>,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sgemm_bf16x3<T, true>,
                                                          T::NT, 0) != hipSuccess ||
             per_cu < 1) {
             (void)hipGetLastError();
@@ -164,16 +159,13 @@ static hipError_t launch_x3_tail(const Problem& p, hipStream_t s, const Arena* a
         }
     }
     (void)hipGetLastError();
-    const bool vec = vec4_ok(p);
     if (sk.splits > 1) {
         sk.dp_tiles = (int)(tiles - rem);
         const dim3 grid((unsigned)(tiles - rem + rem * sk.splits));
-        if (vec) hipLaunchKernelGGL((k_sgemm_bf16x3<T, true, true>), grid, dim3(T::NT), 0, s, q, sk);
-        else hipLaunchKernelGGL((k_sgemm_bf16x3<T, false, true>), grid, dim3(T::NT), 0, s, q, sk);
+        hipLaunchKernelGGL((k_sgemm_bf16x3<T, true>), grid, dim3(T::NT), 0, s, q, sk);
     } else {
         const dim3 grid((unsigned)tiles);
-        if (vec) hipLaunchKernelGGL((k_sgemm_bf16x3<T, true, false>), grid, dim3(T::NT), 0, s, q, sk);
-        else hipLaunchKernelGGL((k_sgemm_bf16x3<T, false, false>), grid, dim3(T::NT), 0, s, q, sk);
+        hipLaunchKernelGGL((k_sgemm_bf16x3<T, false>), grid, dim3(T::NT), 0, s, q, sk);
     }
     return hipGetLastError();
 }
@@ -204,16 +196,13 @@ static hipError_t launch_x3_streamk(const Problem& p, hipStream_t s, const Arena
                      arena);
     if (e != hipSuccess) return e;
     (void)hipGetLastError();
-    const bool vec = vec4_ok(p);
     if (nwg > 0) {
         sk.dp_tiles = (int)(tiles - rem);
         const dim3 grid((unsigned)(tiles - rem + nwg));
-        if (vec) hipLaunchKernelGGL((k_sgemm_bf16x3_sk<T, true>), grid, dim3(T::NT), 0, s, q, sk);
-        else hipLaunchKernelGGL((k_sgemm_bf16x3_sk<T, false>), grid, dim3(T::NT), 0, s, q, sk);
+        hipLaunchKernelGGL((k_sgemm_bf16x3_sk<T>), grid, dim3(T::NT), 0, s, q, sk);
     } else {
         const dim3 grid((unsigned)tiles);
-        if (vec) hipLaunchKernelGGL((k_sgemm_bf16x3<T, true, false>), grid, dim3(T::NT), 0, s, q, sk);
-        else hipLaunchKernelGGL((k_sgemm_bf16x3<T, false, false>), grid, dim3(T::NT), 0, s, q, sk);
+        hipLaunchKernelGGL((k_sgemm_bf16x3<T, false>), grid, dim3(T::NT), 0, s, q, sk);
     }
     return hipGetLastError();
 }

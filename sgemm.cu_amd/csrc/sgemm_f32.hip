@@ -13,7 +13,7 @@ constexpr int kGroupN = 8;
 // dispatcher hands out workgroups in id order, so the split units start as
 // the whole tiles finish: a partial last round of whole tiles becomes a full
 // round of shorter units.
-template <class T, bool VEC, bool SPLIT>
+template <class T, bool SPLIT>
 __global__ __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(T::WPE)))
 void k_sgemm_f32(Problem P, SplitK sk) {
     static_assert(!SPLIT || T::KG == 1, "k-group tiles run whole tiles only");
@@ -23,7 +23,7 @@ void k_sgemm_f32(Problem P, SplitK sk) {
     if (!SPLIT || b < sk.dp_tiles) {
         tile_coords(xcd_remap(b, SPLIT ? sk.dp_tiles : (int)gridDim.x), P.tiles_m, P.tiles_n,
                     kGroupN, tm, tn);
-        F32TileRunner<T, VEC>::run(P, tm, tn, smem);
+        F32TileRunner<T>::run(P, tm, tn, smem);
     } else {
         const int units = (int)gridDim.x - sk.dp_tiles;
         const int u = xcd_remap(b - sk.dp_tiles, units);
@@ -31,7 +31,7 @@ void k_sgemm_f32(Problem P, SplitK sk) {
         const int split = u / count;
         const int slot = u - split * count;
         tile_coords(sk.dp_tiles + slot, P.tiles_m, P.tiles_n, kGroupN, tm, tn);
-        F32TileRunner<T, VEC>::run(P, tm, tn, smem, &sk, split, slot);
+        F32TileRunner<T>::run(P, tm, tn, smem, &sk, split, slot);
     }
 }
 
@@ -40,7 +40,7 @@ void k_sgemm_f32(Problem P, SplitK sk) {
 // rest runs k-tile iterations [g * iters, (g + 1) * iters) of the remaining
 // tiles, tile-major with k-tiles in order; a tile cut at a workgroup boundary
 // is reduced from its pieces in k order by the last of them to finish.
-template <class T, bool VEC>
+template <class T>
 __global__ __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(T::WPE)))
 void k_sgemm_f32_sk(Problem P, SplitK sk) {
     __shared__ __attribute__((aligned(16))) float smem[T::LDS_FLOATS];
@@ -48,12 +48,12 @@ void k_sgemm_f32_sk(Problem P, SplitK sk) {
     const int b = blockIdx.x;
     if (b < sk.dp_tiles) {
         tile_coords(xcd_remap(b, sk.dp_tiles), P.tiles_m, P.tiles_n, kGroupN, tm, tn);
-        F32TileRunner<T, VEC>::run(P, tm, tn, smem);
+        F32TileRunner<T>::run(P, tm, tn, smem);
         return;
     }
     streamk_pieces(P, sk, b, T::BK, [&](int tile, const KPiece& pc) {
         tile_coords(tile, P.tiles_m, P.tiles_n, kGroupN, tm, tn);
-        F32TileRunner<T, VEC>::run_piece(P, tm, tn, smem, &sk, pc);
+        F32TileRunner<T>::run_piece(P, tm, tn, smem, &sk, pc);
     });
 }
 
@@ -78,6 +78,8 @@ static const ConfigInfo kF32Configs[] = {
     {128, 128, 16, 2, 2, "f32_128x128x16_w2x2_3wpe_splitk", 1},
     {128, 128, 16, 2, 2, "f32_128x128x16_w2x2_3wpe_streamk", 3},
     {64, 64, 64, 2, 2, "f32_64x64x64_w2x2_kg2"},
+    {128, 128, 32, 2, 2, "f32_128x128x32_w2x2_cprefetch"},
+    {128, 64, 32, 2, 2, "f32_128x64x32_w2x2_cprefetch"},
 };
 
 int f32_config_count() { return (int)(sizeof(kF32Configs) / sizeof(kF32Configs[0])); }
@@ -103,13 +105,8 @@ static hipError_t launch_tile(const Problem& p, hipStream_t s, const Arena* aren
     const dim3 grid((unsigned)(tiles * sk.splits));
     (void)hipGetLastError();  // the status read below must belong to this launch
     constexpr bool S = ALLOW_SPLIT;
-    if (vec4_ok(p)) {
-        if (split) hipLaunchKernelGGL((k_sgemm_f32<T, true, S>), grid, dim3(T::NT), 0, s, q, sk);
-        else hipLaunchKernelGGL((k_sgemm_f32<T, true, false>), grid, dim3(T::NT), 0, s, q, sk);
-    } else {
-        if (split) hipLaunchKernelGGL((k_sgemm_f32<T, false, S>), grid, dim3(T::NT), 0, s, q, sk);
-        else hipLaunchKernelGGL((k_sgemm_f32<T, false, false>), grid, dim3(T::NT), 0, s, q, sk);
-    }
+    if (split) hipLaunchKernelGGL((k_sgemm_f32<T, S>), grid, dim3(T::NT), 0, s, q, sk);
+    else hipLaunchKernelGGL((k_sgemm_f32<T, false>), grid, dim3(T::NT), 0, s, q, sk);
     return hipGetLastError();
 }
 
@@ -118,7 +115,7 @@ template <class T>
 static long long resident_workgroups() {
     static const long long n = [] {
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sgemm_f32<T, true, true>,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sgemm_f32<T, true>,
                                                          T::NT, 0) != hipSuccess ||
             per_cu < 1) {
             (void)hipGetLastError();
@@ -167,16 +164,13 @@ static hipError_t launch_tail_split(const Problem& p, hipStream_t s, const Arena
         }
     }
     (void)hipGetLastError();  // the status read below must belong to this launch
-    const bool vec = vec4_ok(p);
     if (sk.splits > 1) {
         sk.dp_tiles = (int)(tiles - rem);
         const dim3 grid((unsigned)(tiles - rem + rem * sk.splits));
-        if (vec) hipLaunchKernelGGL((k_sgemm_f32<T, true, true>), grid, dim3(T::NT), 0, s, q, sk);
-        else hipLaunchKernelGGL((k_sgemm_f32<T, false, true>), grid, dim3(T::NT), 0, s, q, sk);
+        hipLaunchKernelGGL((k_sgemm_f32<T, true>), grid, dim3(T::NT), 0, s, q, sk);
     } else {
         const dim3 grid((unsigned)tiles);
-        if (vec) hipLaunchKernelGGL((k_sgemm_f32<T, true, false>), grid, dim3(T::NT), 0, s, q, sk);
-        else hipLaunchKernelGGL((k_sgemm_f32<T, false, false>), grid, dim3(T::NT), 0, s, q, sk);
+        hipLaunchKernelGGL((k_sgemm_f32<T, false>), grid, dim3(T::NT), 0, s, q, sk);
     }
     return hipGetLastError();
 }
@@ -206,16 +200,13 @@ static hipError_t launch_streamk(const Problem& p, hipStream_t s, const Arena* a
                      arena);
     if (e != hipSuccess) return e;
     (void)hipGetLastError();
-    const bool vec = vec4_ok(p);
     if (nwg > 0) {
         sk.dp_tiles = (int)(tiles - rem);
         const dim3 grid((unsigned)(tiles - rem + nwg));
-        if (vec) hipLaunchKernelGGL((k_sgemm_f32_sk<T, true>), grid, dim3(T::NT), 0, s, q, sk);
-        else hipLaunchKernelGGL((k_sgemm_f32_sk<T, false>), grid, dim3(T::NT), 0, s, q, sk);
+        hipLaunchKernelGGL((k_sgemm_f32_sk<T>), grid, dim3(T::NT), 0, s, q, sk);
     } else {
         const dim3 grid((unsigned)tiles);
-        if (vec) hipLaunchKernelGGL((k_sgemm_f32<T, true, false>), grid, dim3(T::NT), 0, s, q, sk);
-        else hipLaunchKernelGGL((k_sgemm_f32<T, false, false>), grid, dim3(T::NT), 0, s, q, sk);
+        hipLaunchKernelGGL((k_sgemm_f32<T, false>), grid, dim3(T::NT), 0, s, q, sk);
     }
     return hipGetLastError();
 }

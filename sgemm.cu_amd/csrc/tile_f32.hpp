@@ -48,8 +48,12 @@ namespace sgemm_amd {
 // sums are added in group order through LDS before the epilogue.  Two groups
 // put two waves of the tile on each SIMD, so one covers the other's LDS and
 // barrier waits.  Whole tiles only (no split-K).
+// CPF_: with beta != 0, a whole tile's C values are loaded into registers
+// right after its first k-tile's operands, so the read of C overlaps the
+// k loop instead of following it -- for short k, where C is most of the
+// traffic and the loop is too short to hide a read issued in the epilogue.
 template <int BM_, int BN_, int BK_, int WM_, int WN_, bool AT_ = false, int WPE_ = 1,
-          int SCHED_ = 0, int PF_ = 1, int KG_ = 1>
+          int SCHED_ = 0, int PF_ = 1, int KG_ = 1, bool CPF_ = false>
 struct F32Tile {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
     static constexpr bool AT = AT_;
@@ -57,6 +61,7 @@ struct F32Tile {
     static constexpr int SCHED = SCHED_;
     static constexpr int PF = PF_;
     static constexpr int KG = KG_;
+    static constexpr bool CPF = CPF_;
     static constexpr int NT = 64 * WM * WN * KG;
     static constexpr int TM = BM / WM, TN = BN / WN;  // wave tile
     static constexpr int MB = TM / 16, NB = TN / 16;  // 16x16 MFMA blocks per wave
@@ -81,20 +86,9 @@ struct F32Tile {
     static_assert(KG >= 1 && BK % (16 * KG) == 0, "k groups split the k-tile's k-steps");
     static_assert(KG == 1 || (KG - 1) * (TM / 16) * (TN / 16) * 64 * WM * WN * 4 <= LDS_FLOATS,
                   "k-group sums fit in LDS");
+    static_assert(!CPF || KG == 1, "C prefetch on one-group tiles");
+    static_assert(epilogue_lds_floats(WM * WN, TM, 16) <= LDS_FLOATS, "epilogue LDS");
 };
-
-// Rows r..r+3 of one column, zero outside [0, valid).  VEC: one 16-byte load
-// when all four are valid.
-template <bool VEC>
-__device__ inline f32x4 load4(const float* p, int valid) {
-    f32x4 v = {0.f, 0.f, 0.f, 0.f};
-    if (VEC && valid >= 4) return *reinterpret_cast<const f32x4*>(p);
-    if (valid > 0) v[0] = p[0];
-    if (valid > 1) v[1] = p[1];
-    if (valid > 2) v[2] = p[2];
-    if (valid > 3) v[3] = p[3];
-    return v;
-}
 
 // Instruction-group order for the compiler's scheduler in a full k-tile
 // (__builtin_amdgcn_sched_group_barrier: mask, count, sync id; 0x008 MFMA,
@@ -131,7 +125,7 @@ __device__ inline void f32_schedule() {
 // body (faster, more registers); the persistent grouped kernel, which holds
 // several tile shapes at once, turns it off to keep its register budget.
 // Both orders do the same arithmetic, so results are identical bits.
-template <class T, bool VEC, bool PEEL = true>
+template <class T, bool PEEL = true>
 struct F32TileRunner {
     static constexpr int BM = T::BM, BN = T::BN, BK = T::BK;
 
@@ -213,9 +207,9 @@ struct F32TileRunner {
             const auto ar = make_rsrc(P.A + a_base, 4 * (a_ext - a_base));
             const auto br = make_rsrc(P.B + b_base, 4 * (b_ext - b_base));
 #pragma unroll
-            for (int r = 0; r < T::A_VEC; ++r) ra[q][r] = bload4<VEC>(ar, a_off[r]);
+            for (int r = 0; r < T::A_VEC; ++r) ra[q][r] = bload4(ar, a_off[r]);
 #pragma unroll
-            for (int r = 0; r < T::B_VEC; ++r) rb[q][r] = bload4<VEC>(br, b_off[r]);
+            for (int r = 0; r < T::B_VEC; ++r) rb[q][r] = bload4(br, b_off[r]);
         };
         // MASK: zero k >= K (only the last k-tile can reach past K).
         auto sstore_impl = [&](float* stage, int k0, auto mask, auto stage_tag) {
@@ -267,8 +261,29 @@ struct F32TileRunner {
         const int kbeg = pc.kbeg;
         const int kend = pc.kend;
         const int ktiles = kend > kbeg ? div_up(kend - kbeg, BK) : 0;
+        // C prefetch (whole tiles with beta != 0): issued after the first
+        // k-tile's operand loads, in the epilogue's read-back mapping
+        // (epilogue_block: lane -> column c_in of each pass, rows r..r+3).
+        constexpr int LPC = T::TM / 4, CPP = 64 / LPC, PASSES = 16 / CPP;
+        constexpr int NCP = T::CPF ? T::NB * PASSES : 1;
+        f32x4 cpre[NCP];
+        const bool prefetch_c = T::CPF && pc.n == 1 && P.beta != 0.f;
+        auto load_c = [&]() {
+            const int row = m0 + wm * T::TM + 4 * (lane % LPC);
+            const int rvalid = min(max(M - row, 0), 4);
+#pragma unroll
+            for (int j = 0; j < T::NB; ++j)
+#pragma unroll
+                for (int p = 0; p < PASSES; ++p) {
+                    const int col = n0 + wn * T::TN + 16 * j + p * CPP + lane / LPC;
+                    cpre[(j * PASSES + p) % NCP] = load4(
+                        P.C + (size_t)min(col, N - 1) * P.ldc + row, col < N ? rvalid : 0);
+                }
+        };
         if (ktiles > 0) {
             gload(kbeg, Stage0{});
+            if constexpr (T::CPF)
+                if (prefetch_c) load_c();
             if constexpr (T::PF > 1) {
 #pragma unroll
                 for (int q = 1; q < T::PF; ++q) {
@@ -446,36 +461,20 @@ struct F32TileRunner {
             }
         }
 
-        // ---- epilogue: lane holds rows 4g..4g+3 of column l16 of each block.
-        const float alpha = P.alpha, beta = P.beta;
+        // ---- epilogue through LDS (epilogue_block): whole 16-byte pieces
+        // down each column of C.  The k loop's last barrier (or the split
+        // reduction's) has freed the LDS.
+        float* elds = smem + kEpilogueLdsSkip + wv * 16 * (T::TM + 4);
 #pragma unroll
-        for (int j = 0; j < T::NB; ++j) {
-            const int col = n0 + wn * T::TN + 16 * j + l16;
-            if (col >= N) continue;
-            float* cc = P.C + (size_t)col * P.ldc;
-#pragma unroll
-            for (int i = 0; i < T::MB; ++i) {
-                const int row = m0 + wm * T::TM + 16 * i + 4 * g;
-                const int valid = min(max(M - row, 0), 4);
-                if (valid == 0) continue;
-                f32x4 v;
-                if (beta != 0.f) {
-                    const f32x4 c = load4<VEC>(cc + row, valid);
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) v[r] = __builtin_fmaf(beta, c[r], alpha * acc[i][j][r]);
-                } else {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) v[r] = alpha * acc[i][j][r];
-                }
-                if (VEC && valid == 4) {
-                    *reinterpret_cast<f32x4*>(cc + row) = v;
-                } else {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        if (r < valid) cc[row + r] = v[r];
-                }
-            }
-        }
+        for (int j = 0; j < T::NB; ++j)
+            epilogue_block<T::TM, 16, T::MB>(
+                elds, P, m0 + wm * T::TM, n0 + wn * T::TN + 16 * j,
+                [&](int i, int) { return acc[i][j]; },
+                [&](int p, const float* cc, int valid) -> f32x4 {
+                    if constexpr (T::CPF)
+                        if (prefetch_c && ktiles > 0) return cpre[(j * PASSES + p) % NCP];
+                    return load4(cc, valid);
+                });
     }
 };
 
@@ -492,6 +491,11 @@ using F32_128x128_k16 = F32Tile<128, 128, 16, 2, 2, false, 3, 1>;
 // 64x64x64 with two k groups of four waves, loads two k-tiles ahead: for
 // grids of one 64x64 tile per CU (1024^3: 86 -> 92 TFLOP/s).
 using F32_64x64_kg2 = F32Tile<64, 64, 64, 2, 2, false, 1, 0, 2, 2>;
+// Short k (a few k-tiles): C prefetched at the start of the tile, so its
+// read overlaps the k loop.  (Loading two k-tiles ahead as well spills on
+// these tiles: 128x64 27, 128x128 84 VGPRs even without the C registers.)
+using F32_128x128_short = F32Tile<128, 128, 32, 2, 2, false, 2, 0, 1, 1, true>;
+using F32_128x64_short = F32Tile<128, 64, 32, 2, 2, false, 3, 0, 1, 1, true>;
 
 template <class T>
 struct TypeTag {
@@ -508,6 +512,8 @@ __host__ __device__ inline bool dispatch_f32(int id, F&& f) {
         case 3: f(TypeTag<F32_256x128>{}); return true;
         case 4: f(TypeTag<F32_128x128_at>{}); return true;
         case 5: f(TypeTag<F32_128x128_k16>{}); return true;
+        case 13: f(TypeTag<F32_128x128_short>{}); return true;
+        case 14: f(TypeTag<F32_128x64_short>{}); return true;
         default: return false;
     }
 }

@@ -78,18 +78,8 @@ struct X3Tile {
     static_assert(A_SUB >= 1 && (BM / 4) * (BK / 4) % NT == 0, "A split");
     static_assert(B_VEC >= 1 && (BN * BK / 4) % NT == 0, "B split");
     static_assert(LDS_BYTES <= 160 * 1024, "LDS budget");
+    static_assert(epilogue_lds_floats(WM * WN, TM, MF) * 4 <= LDS_BYTES, "epilogue LDS");
 };
-
-template <bool VEC>
-__device__ inline f32x4 x3_load4(const float* p, int valid) {
-    f32x4 v = {0.f, 0.f, 0.f, 0.f};
-    if (VEC && valid >= 4) return *reinterpret_cast<const f32x4*>(p);
-    if (valid > 0) v[0] = p[0];
-    if (valid > 1) v[1] = p[1];
-    if (valid > 2) v[2] = p[2];
-    if (valid > 3) v[3] = p[3];
-    return v;
-}
 
 // hi/lo split of four values (round to nearest even both times), two at a
 // time so each rounding is one v_cvt_pk_bf16_f32 for a pair.
@@ -128,7 +118,7 @@ __device__ inline void split4(f32x4 x, bf16x4_t& hi, bf16x4_t& lo) {
 // body (faster, more registers); the persistent grouped kernel, which holds
 // several tile shapes at once, turns it off to keep its register budget.
 // Both orders do the same arithmetic, so results are identical bits.
-template <class T, bool VEC, bool PEEL = true>
+template <class T, bool PEEL = true>
 struct X3TileRunner {
     static constexpr int BM = T::BM, BN = T::BN, BK = T::BK;
 
@@ -191,9 +181,9 @@ struct X3TileRunner {
 #pragma unroll
             for (int r = 0; r < T::A_SUB; ++r)
 #pragma unroll
-                for (int kk = 0; kk < 4; ++kk) ra[r][kk] = bload4<VEC>(ar, a_off[r] + 4 * kk * P.lda);
+                for (int kk = 0; kk < 4; ++kk) ra[r][kk] = bload4(ar, a_off[r] + 4 * kk * P.lda);
 #pragma unroll
-            for (int r = 0; r < T::B_VEC; ++r) rb[r] = bload4<VEC>(br, b_off[r]);
+            for (int r = 0; r < T::B_VEC; ++r) rb[r] = bload4(br, b_off[r]);
         };
         // MASK: zero k >= K (only the last k-tile can reach past K).
         auto sstore_impl = [&](char* stage, int k0, auto mask) {
@@ -410,47 +400,26 @@ struct X3TileRunner {
             }
         }
 
-        // Epilogue.  16x16 block: lane holds rows 4*(lane/16) + 0..3 of column
-        // lane % 16.  32x32 block: register group q (4 registers) holds rows
-        // 8q + 4*(lane/32) + 0..3 of column lane % 32.
-        const float alpha = P.alpha, beta = P.beta;
-        constexpr int GROUPS = T::MF == 16 ? 1 : 4;
+        // Epilogue through LDS (epilogue_block): whole 16-byte pieces down
+        // each column of C.  16x16 block: lane holds rows 4*(lane/16) + 0..3
+        // of column lane % 16; 32x32 block: register group q (4 registers)
+        // holds rows 8q + 4*(lane/32) + 0..3 of column lane % 32.
+        float* elds = reinterpret_cast<float*>(smem) + kEpilogueLdsSkip +
+                      wave * T::MF * (T::TM + 4);
 #pragma unroll
-        for (int j = 0; j < T::NB; ++j) {
-            const int col = n0 + wn * T::TN + T::MF * j + lrow;
-            if (col >= N) continue;
-            float* cc = P.C + (size_t)col * P.ldc;
-#pragma unroll
-            for (int i = 0; i < T::MB; ++i)
-#pragma unroll
-                for (int q = 0; q < GROUPS; ++q) {
-                    const int row = m0 + wm * T::TM + T::MF * i + 8 * q + 4 * (lane / T::MF);
-                    const int valid = min(max(M - row, 0), 4);
-                    if (valid == 0) continue;
+        for (int j = 0; j < T::NB; ++j)
+            epilogue_block<T::TM, T::MF, T::MB>(
+                elds, P, m0 + wm * T::TM, n0 + wn * T::TN + T::MF * j,
+                [&](int i, int q) {
                     f32x4 s;
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         s[r] = acc[i][j][4 * q + r];
                         if constexpr (T::DUAL) s[r] += crs[i][j][4 * q + r];
                     }
-                    f32x4 v;
-                    if (beta != 0.f) {
-                        const f32x4 c = x3_load4<VEC>(cc + row, valid);
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) v[r] = __builtin_fmaf(beta, c[r], alpha * s[r]);
-                    } else {
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) v[r] = alpha * s[r];
-                    }
-                    if (VEC && valid == 4) {
-                        *reinterpret_cast<f32x4*>(cc + row) = v;
-                    } else {
-#pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            if (r < valid) cc[row + r] = v[r];
-                    }
-                }
-        }
+                    return s;
+                },
+                [&](int, const float* cc, int valid) { return load4(cc, valid); });
     }
 };
 

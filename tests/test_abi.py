@@ -43,6 +43,11 @@ def test_invalid_arguments_rejected_on_host(lib):
         assert sa.sgemm_bf16x3(m, n, k, 1.0, p, lda, p, ldb, 0.0, p, ldc, check=False) == 1
     assert sa.sgemm(4, 4, 4, 1.0, None, 4, p, 4, 0.0, p, 4) == 1
     assert sa.sgemm(4, 4, 4, 1.0, p, 4, p, 4, 0.0, None, 4) == 1
+    # operands that are not 4-byte aligned (no float pointer is): rejected
+    for A, B, C in ((p + 2, p, p), (p, p + 1, p), (p, p, p + 3)):
+        assert sa.sgemm_ex(4, 4, 4, 1.0, A, 4, B, 4, 0.0, C, 4, check=False) == 1
+        assert sa.sgemm_bf16x3(4, 4, 4, 1.0, A, 4, B, 4, 0.0, C, 4, check=False) == 1
+        assert sa.sgemm_ex_noalloc(0, 4, 4, 4, 1.0, A, 4, B, 4, 0.0, C, 4, check=False) == 1
     # null alpha / beta pointers
     L = sa.lib()
     assert L.sgemm_ex(4, 4, 4, None, p, 4, p, 4, None, p, 4, None) == 1

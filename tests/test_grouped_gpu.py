@@ -150,6 +150,10 @@ def test_grouped_rejects_bad_problem_and_capacity():
     with pytest.raises(sa.SgemmError) as e:
         plan.set(bad)
     assert e.value.status == 1
+    misaligned = [sa.problem(4, 4, 4, 1.0, C.data_ptr() + 2, 4, C, 4, 0.0, C, 4)]
+    with pytest.raises(sa.SgemmError) as e:
+        plan.set(misaligned)  # not 4-byte aligned: no float pointer is
+    assert e.value.status == 1
     ok = sa.problem(4, 4, 4, 1.0, C, 4, C, 4, 0.0, C, 4)
     with pytest.raises(sa.SgemmError):
         plan.set([ok, ok, ok])  # over capacity

@@ -24,7 +24,7 @@ from gpu_util import (Case, assert_ok, check, compact, download, log, padding_un
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gemm_cases.json")
-N_F32_CONFIGS = 13
+N_F32_CONFIGS = 15
 
 
 @pytest.fixture(scope="module")
@@ -202,8 +202,9 @@ def test_rowmajor_shim_matches_reference_semantics():
     assert np.all(np.abs(got - want) <= orc.fp32_tol_mult(k) * scale)
 
 
-def test_unaligned_operands_take_scalar_path():
-    """Odd leading dimensions and 4-byte-offset pointers (no 16-byte vectors)."""
+def test_odd_leading_dimensions_and_offset_pointers():
+    """Odd leading dimensions and pointers one float past a 16-byte boundary:
+    the 16-byte loads and stores need only dword alignment."""
     import torch
     m, n, k = 131, 67, 77
     c = Case(m, n, k, lda=m + 2, ldb=k + 1, ldc=m + 2, seed=17)
