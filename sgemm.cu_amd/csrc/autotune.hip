@@ -84,10 +84,12 @@ int heuristic(int form, int grouped, int m, int n, int k) {
         // k for the tiles of a partial last round where the model of
         // tail_split_count says it pays, and is config 5 elsewhere:
         // 3584^3 104 -> 136, 4608^3 117 -> 133, 5120^3 123 -> 139,
-        // 9216^3 135 -> 141 TFLOP/s.  Short k (<= 256) is epilogue-bound:
-        // 128x64 (config 2).
+        // 9216^3 135 -> 141 TFLOP/s.  Short k (<= 256) is bound by reading
+        // and writing C: 128x64 with C prefetched at the start of each tile
+        // (config 14): 8192x8192x127 (alpha 1.5, beta 0.5) 76 -> 94,
+        // 8192x8192x256 109 -> 116 TFLOP/s over config 2.
         const long long t128 = (long long)div_up(m, 128) * div_up(n, 128);
-        if (k <= 256 && t128 >= 512) return 2;
+        if (k <= 256 && t128 >= 512) return 14;
         if (t128 >= 700) return 9;
         // From 350 tiles with k >= 1024: config 5's tile with stream-K over
         // all tiles (config 11; no full round of 768 workgroups exists here):
