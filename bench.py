@@ -72,6 +72,13 @@ def parse():
                     help="size sweep instead of the single-size line: square 1024..16384 step "
                          "1024 plus (1000,1000,1000), (4097,3071,2049), (8192,8192,127); "
                          "alpha=1.5, beta=0.5; both forms")
+    ap.add_argument("--shape", type=str, default="",
+                    help="M,N,K: time only the default entry point of --form at this shape "
+                         "(alpha=1.5, beta=0.5, ld = rows; profiling runs) and exit")
+    ap.add_argument("--form", type=str, default="f32", choices=("f32", "bf16x3"),
+                    help="form timed by --shape")
+    ap.add_argument("--grouped-only", action="store_true",
+                    help="run only the grouped leg (profiling runs) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -642,6 +649,36 @@ def main():
     if args.sweep:
         sa.lib()
         sweep(sa, torch, args.warmup)
+        return
+    if args.shape:
+        sa.lib()
+        m, n, k = (int(v) for v in args.shape.split(","))
+        form = sa.FORM_F32 if args.form == "f32" else sa.FORM_BF16X3
+        fn = sa.sgemm_ex if form == sa.FORM_F32 else sa.sgemm_bf16x3
+        A = torch.rand(m * k, device="cuda") * 2 - 1
+        B = torch.rand(k * n, device="cuda") * 2 - 1
+        C = torch.rand(m * n, device="cuda") * 2 - 1
+        stream = torch.cuda.current_stream()
+
+        def call():
+            fn(m, n, k, 1.5, A, m, B, k, 0.5, C, m, stream)
+        # Warm up for about 0.3 s so the engine clock has settled (a few
+        # calls of a short problem end before it rises), then time `steps`.
+        probe = time_form(call, 3, 1, stream)
+        time_form(call, max(args.warmup, int(300.0 / max(probe, 1e-3))), 0, stream)
+        ms = time_form(call, args.steps, 0, stream)
+        print(json.dumps({"shape": [m, n, k], "form": args.form, "ms": ms,
+                          "tflops": 2.0 * m * n * k / (ms * 1e-3) / 1e12,
+                          "config": sa.autotune_lookup(form, m, n, k)[0]}), flush=True)
+        return
+    if args.grouped_only:
+        sa.lib()
+        stream = torch.cuda.current_stream()
+        out = {}
+        for form, name in ((sa.FORM_F32, "f32"), (sa.FORM_BF16X3, "bf16x3")):
+            out[name] = grouped_leg(sa, torch, form, args.group_count, args.steps, args.warmup,
+                                    stream, tune_iters=args.tune_iters)
+        print(json.dumps({"grouped": out}), flush=True)
         return
     sa.lib()
     torch.manual_seed(0)
