@@ -212,8 +212,12 @@ int sgemm_grouped(int count, const sgemm_problem* problems, int form, sgemm_stre
 /* ------------------------------------------------------------------------ */
 /* Autotuner (no reference counterpart): one tile config per (form, grouped, */
 /* size bucket of m, n, k), bucket(x) = ceil(log2 x) capped to [0, 15].      */
-/* Selection is a pure function of the shape and the loaded table.  A table  */
-/* named by the environment variable SGEMM_AMD_TUNE_TABLE loads on first use.*/
+/* Selection is a pure function of the shape, the loaded table and, for the */
+/* FP32 heuristic, whether every column of A and B is 16-byte aligned (A, B */
+/* 16-byte aligned, lda and ldb multiples of 4: configs 5, 9 and 10 then    */
+/* load their operands straight into LDS and hold four workgroups per CU).  */
+/* A table named by the environment variable SGEMM_AMD_TUNE_TABLE loads on  */
+/* first use.                                                               */
 /* ------------------------------------------------------------------------ */
 
 /* Time every candidate config on an (m, n, k) problem (`iters` launches each,
@@ -221,8 +225,9 @@ int sgemm_grouped(int count, const sgemm_problem* problems, int form, sgemm_stre
  * Allocates its own operands; not for hot paths. */
 int sgemm_autotune_bucket(int form, int grouped, int m, int n, int k, int iters,
                           sgemm_stream_t stream, int* best_config, float* best_tflops);
-/* Config the library will use for this shape; *from_table = 1 if a table
- * entry (not the built-in heuristic) decided. */
+/* Config the library will use for this shape (with 16-byte aligned operand
+ * columns); *from_table = 1 if a table entry (not the built-in heuristic)
+ * decided. */
 int sgemm_autotune_lookup(int form, int grouped, int m, int n, int k, int* config_id,
                           int* from_table);
 /* Set one bucket's config by hand. */

@@ -97,7 +97,7 @@ int sgemm_amd::run_problem(int form, int config_id, int m, int n, int k, const f
         if (config_id >= 0) return SGEMM_STATUS_NOT_SUPPORTED;
         return hip_status(launch_wide(p, stream));
     }
-    if (config_id < 0) config_id = choose_config(form, m, n, k);
+    if (config_id < 0) config_id = choose_config(form, m, n, k, columns_16b_aligned(p));
     hipError_t e = form == SGEMM_FORM_F32 ? launch_f32(config_id, p, stream, arena)
                                           : launch_bf16x3(config_id, p, stream, arena);
     // With an arena nothing is allocated: out of memory means the plan needs

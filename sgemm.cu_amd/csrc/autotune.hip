@@ -71,45 +71,52 @@ void ensure_env_table(Table& t) {
     if (p && *p) load_locked(t, p);
 }
 
-int heuristic(int form, int grouped, int m, int n, int k) {
+int heuristic(int form, int grouped, int m, int n, int k, bool aligned) {
     if (form == kFormF32) {
         if (grouped) {
             if (m >= 96 && n >= 96) return 5;
             return m >= 96 ? 2 : 1;
         }
-        // Measured on MI355X (square sweeps 1024..16384, the sizes whose
-        // last round is short, and the odd shapes).  128x128x16 with three
-        // workgroups per CU (768 resident) is the fastest tile from about
-        // 700 tiles of 128x128 on.  Its tail-split form (config 9) splits
-        // k for the tiles of a partial last round where the model of
-        // tail_split_count says it pays, and is config 5 elsewhere:
-        // 3584^3 104 -> 136, 4608^3 117 -> 133, 5120^3 123 -> 139,
-        // 9216^3 135 -> 141 TFLOP/s.  Short k (<= 256) is bound by reading
-        // and writing C: 128x64 with C prefetched at the start of each tile
-        // (config 14): 8192x8192x127 (alpha 1.5, beta 0.5) 76 -> 94,
-        // 8192x8192x256 109 -> 116 TFLOP/s over config 2.
+        // Measured on MI355X (square sweeps 1024..8192 step 256, with
+        // aligned and with odd leading dimensions, the sizes whose last
+        // round is short, and the odd shapes).  From about one round of
+        // 128x128 tiles on, config 5's tile is the fastest, with its tail
+        // split (config 9) cutting k for the tiles of a partial last round
+        // where tail_split_count's model says it pays: 4352^3 128 -> 140,
+        // 4608^3 122 -> 139, 5888^3 133 -> 145, 7168^3 137 -> 146 TFLOP/s
+        // (direct-loaded tile, 1024 resident).  The register-staged tile
+        // (768 resident) takes config 9 from 700 tiles: 3584^3 105 -> 137.
+        // Short k (<= 256) is bound by reading and writing C: 128x64 with
+        // C prefetched at the start of each tile (config 14): 8192x8192x127
+        // (alpha 1.5, beta 0.5) 76 -> 98, 8192x8192x256 109 -> 116 TFLOP/s
+        // over config 2.
         const long long t128 = (long long)div_up(m, 128) * div_up(n, 128);
         if (k <= 256 && t128 >= 512) return 14;
-        if (t128 >= 700) return 9;
-        // From 350 tiles with k >= 1024: config 5's tile with stream-K over
-        // all tiles (config 11; no full round of 768 workgroups exists here):
-        // 2432^3 120 -> 124, 2560^3 121 -> 123, 2816^3 123 -> 128 TFLOP/s,
-        // 8192x1024x8192 132 -> 133.
-        if (k >= 1024 && t128 >= 350) return 11;
-        // Below that, 64x64 tiles (config 7: loads two k-tiles ahead,
+        if (aligned) {
+            if (t128 >= 1024) return 9;
+            // 700-1023 tiles, short of a round of 1024: stream-K over all
+            // tiles (config 11, register-staged, 768 resident) for k >= 1024
+            // (3584^3 110 -> 131, 5120x2560x4096 113 -> 134 TFLOP/s), else
+            // 64x64 tiles (3584x3584x512 104 -> 114, 4096x3584x768 121 ->
+            // 125).
+            if (t128 >= 700) return k >= 1024 ? 11 : 7;
+        } else if (t128 >= 700) {
+            return 9;
+        }
+        // Below that with k >= 1024, from 180 tiles of 128x128: with t 64x64
+        // tiles per CU on average the busiest CU runs ceil(t).  Where that
+        // loads the CUs unevenly, config 5's tile with split-K (config 10)
+        // is faster, else 64x64 tiles (config 7: loads two k-tiles ahead,
         // split-K when the grid holds fewer than about four workgroups per
-        // CU) unless their tiles load the CUs unevenly: with t tiles per CU
-        // on average, the busiest CU runs ceil(t).  Under 90 % of that, and
-        // from 180 tiles of 128x128 with k >= 1024, config 5's tile with
-        // split-K (config 10) is faster (square sweep 1024..5120; 2560^3 and
-        // 3328^3 now take config 11):
-        //   1792^3 (3.1 64x64 tiles per CU) 96 -> 105, 2304^3 (5.1) 108 ->
-        //   119, 2560^3 (6.25) 115 -> 121 TFLOP/s;
-        // while evenly loaded sizes keep 64x64: 2688^3 (6.9) 127 vs 112,
-        // 3072^3 (9.0) 131 vs 120, 3328^3 (10.6) 128 vs 119.
+        // CU).  The threshold is 0.9 below 350 tiles of 128x128, 0.95 from
+        // there: 1792^3 (3.1 64x64 tiles per CU) 94 -> 102, 2304^3 (5.1)
+        // 106 -> 124, 2560^3 (6.25) 114 -> 124, 2816^3 (7.6) 122 -> 131
+        // TFLOP/s; while 3072^3 (9.0) 128 vs 124 and 3328^3 (10.6) 125 vs
+        // 124 keep 64x64 (stream-K, config 11, ran 116-123 on these).
         if (k >= 1024 && t128 >= 180) {
             const double t = (double)div_up(m, 64) * div_up(n, 64) / device_cu_count();
-            if (t / std::ceil(t) < 0.9) return 10;
+            if (t / std::ceil(t) < (t128 >= 350 ? 0.95 : 0.9)) return 10;
+            if (t128 >= 350) return 7;
         }
         // One 64x64 tile per CU (90-100 % of the CUs) and k <= 1536: the
         // k-group tile (config 12) over split-K: 1024^3 86 -> 92, 1000^3 79
@@ -196,7 +203,7 @@ int heuristic(int form, int grouped, int m, int n, int k) {
     return best;
 }
 
-int lookup(int form, int grouped, int m, int n, int k) {
+int lookup(int form, int grouped, int m, int n, int k, bool aligned = true) {
     Table& t = table();
     std::lock_guard<std::mutex> lock(t.mu);
     ensure_env_table(t);
@@ -205,7 +212,7 @@ int lookup(int form, int grouped, int m, int n, int k) {
         const int c = it->second.config;
         if (c >= 0 && c < config_count(form) && (!grouped || grouped_capable(form, c))) return c;
     }
-    return heuristic(form, grouped, m, n, k);
+    return heuristic(form, grouped, m, n, k, aligned);
 }
 
 int load_locked(Table& t, const char* path) {
@@ -260,7 +267,9 @@ int size_bucket(int x) {
     return b > 15 ? 15 : b;
 }
 
-int choose_config(int form, int m, int n, int k) { return lookup(form, 0, m, n, k); }
+int choose_config(int form, int m, int n, int k, bool aligned) {
+    return lookup(form, 0, m, n, k, aligned);
+}
 int choose_grouped_config(int form, int m, int n, int k) { return lookup(form, 1, m, n, k); }
 
 }  // namespace sgemm_amd

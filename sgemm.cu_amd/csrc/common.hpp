@@ -128,6 +128,19 @@ __device__ inline f32x4 bload4(__amdgpu_buffer_rsrc_t r, int off) {
     return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
 
+// 16 bytes per lane from byte offset `off` of buffer `r` into LDS: lane i's
+// data lands at lds + 4 * i floats (buffer_load_dwordx4 with the lds bit;
+// `lds` must be the same for every lane).  Out-of-range dwords write 0.
+// Completion is counted by vmcnt, not lgkmcnt: see wait_vmem.
+__device__ inline void buffer_load_lds16(__amdgpu_buffer_rsrc_t r, float* lds, int off) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+        r, (__attribute__((address_space(3))) void*)lds, 16, off, 0, 0, 0);
+}
+
+// s_waitcnt vmcnt(0): every vector memory load of this wave has completed,
+// including the LDS writes of buffer_load_lds16.
+__device__ inline void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Rows r..r+3 of one column, zero outside [0, valid): one 16-byte load when
 // all four are valid (dword alignment suffices, see bload4).
 __device__ inline f32x4 load4(const float* p, int valid) {

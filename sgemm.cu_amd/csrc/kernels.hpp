@@ -172,4 +172,11 @@ inline const ConfigInfo& config_info(int form, int id) {
 // alignment only, whatever the leading dimensions.
 inline bool dword_aligned(const void* q) { return (reinterpret_cast<uintptr_t>(q) & 3u) == 0; }
 
+// Every column of A and B starts on a 16-byte boundary: the condition for the
+// FP32 direct-to-LDS tile (configs 5, 9 and 10 run it when this holds).
+inline bool columns_16b_aligned(const Problem& p) {
+    return ((reinterpret_cast<uintptr_t>(p.A) | reinterpret_cast<uintptr_t>(p.B)) & 15u) == 0 &&
+           p.lda % 4 == 0 && p.ldb % 4 == 0;
+}
+
 }  // namespace sgemm_amd

@@ -79,7 +79,7 @@ static const ConfigInfo kF32Configs[] = {
     {128, 128, 16, 2, 2, "f32_128x128x16_w2x2_3wpe_streamk", 3},
     {64, 64, 64, 2, 2, "f32_64x64x64_w2x2_kg2"},
     {128, 128, 32, 2, 2, "f32_128x128x32_w2x2_cprefetch"},
-    {128, 64, 32, 2, 2, "f32_128x64x32_w2x2_cprefetch"},
+    {128, 64, 16, 2, 2, "f32_128x64x16_w2x2_cprefetch"},
 };
 
 int f32_config_count() { return (int)(sizeof(kF32Configs) / sizeof(kF32Configs[0])); }
@@ -110,13 +110,16 @@ static hipError_t launch_tile(const Problem& p, hipStream_t s, const Arena* aren
     return hipGetLastError();
 }
 
-// Workgroups of kernel K resident on the device at once.
-template <class T>
+// Workgroups of the split (SK: stream-K) kernel of tile T resident on the
+// device at once.
+template <class T, bool SK = false>
 static long long resident_workgroups() {
     static const long long n = [] {
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sgemm_f32<T, true>,
-                                                         T::NT, 0) != hipSuccess ||
+        void (*kernel)(Problem, SplitK) = nullptr;
+        if constexpr (SK) kernel = k_sgemm_f32_sk<T>;
+        else kernel = k_sgemm_f32<T, true>;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, T::NT, 0) != hipSuccess ||
             per_cu < 1) {
             (void)hipGetLastError();
             per_cu = 1;
@@ -190,7 +193,7 @@ static hipError_t launch_streamk(const Problem& p, hipStream_t s, const Arena* a
     const long long tiles = (long long)q.tiles_m * q.tiles_n;
     if (tiles <= 0) return hipSuccess;
     if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
-    const long long slots = resident_workgroups<T>();
+    const long long slots = resident_workgroups<T, true>();
     const long long rem = tiles % slots;
     const size_t split_bytes = (size_t)T::MB * T::NB * T::NT * 16;
     SplitK sk{nullptr, nullptr, 1, p.k, 0};
@@ -213,12 +216,23 @@ static hipError_t launch_streamk(const Problem& p, hipStream_t s, const Arena* a
 
 hipError_t launch_f32(int id, const Problem& p, hipStream_t s, const Arena* arena) {
     if (id < 0 || id >= f32_config_count()) return hipErrorInvalidValue;
+    const bool dl = columns_16b_aligned(p);
     switch (id) {
+        case 5:
+            return dl ? launch_tile<F32_128x128_k16_dl, false>(p, s, arena)
+                      : launch_tile<F32_128x128_k16, false>(p, s, arena);
         case 6: return launch_tile<F32_128x128, true>(p, s, arena);
         case 7: return launch_tile<F32_64x64, true>(p, s, arena);
         case 8: return launch_tile<F32_128x64, true>(p, s, arena);
-        case 9: return launch_tail_split<F32_128x128_k16>(p, s, arena);
-        case 10: return launch_tile<F32_128x128_k16, true, 8>(p, s, arena);
+        case 9:
+            return dl ? launch_tail_split<F32_128x128_k16_dl>(p, s, arena)
+                      : launch_tail_split<F32_128x128_k16>(p, s, arena);
+        case 10:
+            return dl ? launch_tile<F32_128x128_k16_dl, true, 8>(p, s, arena)
+                      : launch_tile<F32_128x128_k16, true, 8>(p, s, arena);
+        // Stream-K keeps the register-staged tile: with the direct-loaded
+        // one (at three workgroups per CU, as its piece loop needs more than
+        // 128 VGPRs) it ran 118 against 130 TFLOP/s at 3584^3.
         case 11: return launch_streamk<F32_128x128_k16>(p, s, arena);
         case 12: return launch_tile<F32_64x64_kg2, false>(p, s, arena);
         default: break;

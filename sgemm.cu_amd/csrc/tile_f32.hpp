@@ -52,8 +52,21 @@ namespace sgemm_amd {
 // right after its first k-tile's operands, so the read of C overlaps the
 // k loop instead of following it -- for short k, where C is most of the
 // traffic and the loop is too short to hide a read issued in the epilogue.
+// DL_: full k-tiles go from HBM straight into LDS (buffer_load_dwordx4 with
+// the lds bit: each lane's 16 bytes land at the instruction's LDS base +
+// 16 * lane), so no registers stage them and no LDS stores follow the loads;
+// the wave waits for its own loads before the k-tile's closing barrier.  The
+// placement is lane-contiguous, so the layout is chosen through which HBM
+// address each lane reads: A as pairs of k rows (2*BM floats, m-contiguous)
+// with 8 floats of padding after each pair, B as rows of BK k-values per
+// column with the 16-byte chunks of row n rotated by XOR with (n / 4) % 4.
+// Both make the fragment reads bank-conflict free.  A k-tile that reaches
+// past K goes through registers (masked) into the same layout, so the
+// arithmetic and the result bits are those of the register-staged tile.
+// Needs 16-byte aligned operand columns (A, B 16-byte aligned, lda and ldb
+// multiples of 4); the launcher picks the register-staged tile otherwise.
 template <int BM_, int BN_, int BK_, int WM_, int WN_, bool AT_ = false, int WPE_ = 1,
-          int SCHED_ = 0, int PF_ = 1, int KG_ = 1, bool CPF_ = false>
+          int SCHED_ = 0, int PF_ = 1, int KG_ = 1, bool CPF_ = false, bool DL_ = false>
 struct F32Tile {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
     static constexpr bool AT = AT_;
@@ -62,13 +75,16 @@ struct F32Tile {
     static constexpr int PF = PF_;
     static constexpr int KG = KG_;
     static constexpr bool CPF = CPF_;
+    static constexpr bool DL = DL_;
     static constexpr int NT = 64 * WM * WN * KG;
     static constexpr int TM = BM / WM, TN = BN / WN;  // wave tile
     static constexpr int MB = TM / 16, NB = TN / 16;  // 16x16 MFMA blocks per wave
-    // A layout: As[k][m] with row stride BM+4, or As[m][k] with row stride BK+4.
+    // A layout: As[k][m] with row stride BM+4, As[m][k] with row stride BK+4
+    // (AT), or pairs of k rows with stride APAIR (DL).
     static constexpr int SA = AT ? BK + 4 : BM + 4;
-    static constexpr int A_FLOATS = AT ? BM * SA : BK * SA;
-    static constexpr int SB = BK + 4;                 // Bs[n][k] row stride (floats)
+    static constexpr int APAIR = 2 * BM + 8;
+    static constexpr int A_FLOATS = DL ? (BK / 2) * APAIR : AT ? BM * SA : BK * SA;
+    static constexpr int SB = DL ? BK : BK + 4;       // Bs[n][k] row stride (floats)
     static constexpr int STAGE = A_FLOATS + BN * SB;  // floats per LDS stage
     static constexpr int LDS_FLOATS = 2 * STAGE;
     static constexpr int A_VEC = BM * BK / 4 / NT;  // float4 of A per thread per k-tile
@@ -88,6 +104,13 @@ struct F32Tile {
                   "k-group sums fit in LDS");
     static_assert(!CPF || KG == 1, "C prefetch on one-group tiles");
     static_assert(epilogue_lds_floats(WM * WN, TM, 16) <= LDS_FLOATS, "epilogue LDS");
+    // DL: one load instruction fills one pair of A rows (BM = 128: 64 lanes x
+    // 4 floats x 2 rows) or 16 B rows of 16 k (BK = 16); each wave issues
+    // A_VEC of the first kind and B_VEC of the second per k-tile.
+    static_assert(!DL || (BM == 128 && BK == 16 && !AT && PF == 1 && KG == 1 && !CPF &&
+                          SCHED == 0 && A_VEC * WM * WN == BK / 2 &&
+                          B_VEC * WM * WN == BN / 16),
+                  "direct-to-LDS tile shape");
 };
 
 // Instruction-group order for the compiler's scheduler in a full k-tile
@@ -181,17 +204,31 @@ struct F32TileRunner {
         const int b_k = 4 * (t % S);
         const int b_q = t / 16;
         const int b_n = (b_q % S) + S * ((t / S) % R) + 16 * (b_q / S);
+        // DL: load instruction r of this wave fills LDS block
+        // wave * A_VEC + r of A (wave * B_VEC + r of B).  A: lanes 0-31 read k row 2 * block, lanes 32-63 row
+        // 2 * block + 1, four m each.  B: lane / 4 is the column within the
+        // block's 16, lane % 4 the chunk position, which holds k-chunk
+        // (lane % 4) ^ ((lane / 16) % 4).
+        const int dl_wave = __builtin_amdgcn_readfirstlane(wave);
+        const int dl_row = lane >> 2, dl_kc = (lane & 3) ^ ((dl_row >> 2) & 3);
         int a_off[T::A_VEC], b_off[T::B_VEC];
 #pragma unroll
         for (int r = 0; r < T::A_VEC; ++r) {
-            if constexpr (T::AT)
+            if constexpr (T::DL)
+                a_off[r] = 4 * (4 * (lane & 31) + (2 * (dl_wave * T::A_VEC + r) + (lane >> 5)) * P.lda);
+            else if constexpr (T::AT)
                 a_off[r] = 4 * (4 * (at_m4 + (r / 4) * (T::NT / AS)) +
                                 (4 * at_k4 + r % 4) * P.lda);
             else
                 a_off[r] = 4 * (a_m + (a_k + r * T::A_KSTEP) * P.lda);
         }
 #pragma unroll
-        for (int r = 0; r < T::B_VEC; ++r) b_off[r] = 4 * (b_k + (b_n + r * T::B_NSTEP) * P.ldb);
+        for (int r = 0; r < T::B_VEC; ++r) {
+            if constexpr (T::DL)
+                b_off[r] = 4 * (4 * dl_kc + (16 * (dl_wave * T::B_VEC + r) + dl_row) * P.ldb);
+            else
+                b_off[r] = 4 * (b_k + (b_n + r * T::B_NSTEP) * P.ldb);
+        }
         const long long a_ext = extent(M, K, P.lda), b_ext = extent(K, N, P.ldb);
 
         // Raw loads land in ra/rb (register stage k-tile % PF); the k >= K
@@ -211,12 +248,43 @@ struct F32TileRunner {
 #pragma unroll
             for (int r = 0; r < T::B_VEC; ++r) rb[q][r] = bload4(br, b_off[r]);
         };
+        // DL: a full k-tile straight into LDS `stage` (same addresses as gload).
+        auto dma = [&](float* stage, int k0) {
+            const long long a_base = m0 + (long long)k0 * P.lda;
+            const long long b_base = k0 + (long long)n0 * P.ldb;
+            const auto ar = make_rsrc(P.A + a_base, 4 * (a_ext - a_base));
+            const auto br = make_rsrc(P.B + b_base, 4 * (b_ext - b_base));
+#pragma unroll
+            for (int r = 0; r < T::A_VEC; ++r)
+                buffer_load_lds16(ar, stage + (dl_wave * T::A_VEC + r) * T::APAIR, a_off[r]);
+#pragma unroll
+            for (int r = 0; r < T::B_VEC; ++r)
+                buffer_load_lds16(br, stage + T::A_FLOATS + (dl_wave * T::B_VEC + r) * 256, b_off[r]);
+        };
         // MASK: zero k >= K (only the last k-tile can reach past K).
         auto sstore_impl = [&](float* stage, int k0, auto mask, auto stage_tag) {
             constexpr bool MASK = decltype(mask)::value;
             constexpr int q = decltype(stage_tag)::value;
             float* As = stage;
             float* Bs = stage + T::A_FLOATS;
+            if constexpr (T::DL) {
+                // The slots dma() would fill, k >= K zeroed.
+#pragma unroll
+                for (int r = 0; r < T::A_VEC; ++r) {
+                    const bool ok = !MASK || k0 + 2 * (dl_wave * T::A_VEC + r) + (lane >> 5) < K;
+                    *reinterpret_cast<f32x4*>(&As[(dl_wave * T::A_VEC + r) * T::APAIR + 4 * lane]) =
+                        ok ? ra[q][r] : f32x4{0.f, 0.f, 0.f, 0.f};
+                }
+                const int kv = K - (k0 + 4 * dl_kc);
+#pragma unroll
+                for (int r = 0; r < T::B_VEC; ++r) {
+                    f32x4 v;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) v[e] = !MASK || e < kv ? rb[q][r][e] : 0.f;
+                    *reinterpret_cast<f32x4*>(&Bs[(dl_wave * T::B_VEC + r) * 256 + 4 * lane]) = v;
+                }
+                return;
+            }
             if constexpr (T::AT) {
                 const int ka = K - (k0 + 4 * at_k4);  // valid k in this sub-block
 #pragma unroll
@@ -280,7 +348,11 @@ struct F32TileRunner {
                         P.C + (size_t)min(col, N - 1) * P.ldc + row, col < N ? rvalid : 0);
                 }
         };
-        if (ktiles > 0) {
+        if (T::DL && ktiles > 0 && kbeg + BK <= K) {
+            dma(smem, kbeg);
+            wait_vmem();
+            __syncthreads();
+        } else if (ktiles > 0) {
             gload(kbeg, Stage0{});
             if constexpr (T::CPF)
                 if (prefetch_c) load_c();
@@ -315,7 +387,8 @@ struct F32TileRunner {
             float* next_stage = smem + ((kt + 1) & 1) * T::STAGE;
             const bool more = NEXT_FULL || kt + 1 < ktiles;
             const int knext = kbeg + (kt + 1) * BK;
-            if (T::PF == 1 ? more : kt + T::PF < ktiles) gload(kbeg + (kt + T::PF) * BK, p_tag);
+            if constexpr (T::DL && NEXT_FULL) dma(next_stage, knext);
+            else if (T::PF == 1 ? more : kt + T::PF < ktiles) gload(kbeg + (kt + T::PF) * BK, p_tag);
             if constexpr (NEXT_FULL && T::SCHED == 0) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int kq = 0; kq < BK / 16 / T::KG; ++kq) {
@@ -326,7 +399,12 @@ struct F32TileRunner {
                 f32x4 b[T::NB];
 #pragma unroll
                 for (int i = 0; i < T::MB; ++i) {
-                    if constexpr (T::AT) {
+                    if constexpr (T::DL) {
+#pragma unroll
+                        for (int s = 0; s < 4; ++s)
+                            a[i][s] = As[(kk / 2 + 2 * g + s / 2) * T::APAIR + (s % 2) * BM +
+                                         wm * T::TM + 16 * i + l16];
+                    } else if constexpr (T::AT) {
                         a[i] = *reinterpret_cast<const f32x4*>(
                             &As[(wm * T::TM + 16 * i + l16) * T::SA + kk + 4 * g]);
                     } else {
@@ -336,9 +414,14 @@ struct F32TileRunner {
                     }
                 }
 #pragma unroll
-                for (int j = 0; j < T::NB; ++j)
-                    b[j] = *reinterpret_cast<const f32x4*>(
-                        &Bs[(wn * T::TN + 16 * j + l16) * T::SB + kk + 4 * g]);
+                for (int j = 0; j < T::NB; ++j) {
+                    if constexpr (T::DL)
+                        b[j] = *reinterpret_cast<const f32x4*>(
+                            &Bs[(wn * T::TN + 16 * j + l16) * T::SB + 4 * (g ^ ((l16 >> 2) & 3))]);
+                    else
+                        b[j] = *reinterpret_cast<const f32x4*>(
+                            &Bs[(wn * T::TN + 16 * j + l16) * T::SB + kk + 4 * g]);
+                }
 #pragma unroll
                 for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -352,15 +435,20 @@ struct F32TileRunner {
                 // overlap matrix-core work instead of following all of it.
                 if (kq == 0) {
                     if constexpr (NEXT_FULL) {
-                        if constexpr (T::SCHED == 0) __builtin_amdgcn_sched_barrier(0);
-                        sstore_impl(next_stage, knext, std::false_type{}, NextStage{});
-                        if constexpr (T::SCHED == 0) __builtin_amdgcn_sched_barrier(0);
+                        if constexpr (!T::DL) {
+                            if constexpr (T::SCHED == 0) __builtin_amdgcn_sched_barrier(0);
+                            sstore_impl(next_stage, knext, std::false_type{}, NextStage{});
+                            if constexpr (T::SCHED == 0) __builtin_amdgcn_sched_barrier(0);
+                        }
                     } else if (more) {
                         sstore(next_stage, knext, NextStage{});
                     }
                 }
             }
             if constexpr (NEXT_FULL && T::SCHED != 0) f32_schedule<T>();
+            // DL: this wave's loads of the next k-tile have reached LDS
+            // before the caller's barrier.
+            if constexpr (T::DL && NEXT_FULL) wait_vmem();
         };
         // k-tiles whose successor is full run the branch-free body, the last
         // one or two the general body (two loops in sequence, so the
@@ -488,14 +576,22 @@ using F32_128x64 = F32Tile<128, 64, 32, 2, 2>;
 using F32_256x128 = F32Tile<256, 128, 16, 2, 2>;
 using F32_128x128_at = F32Tile<128, 128, 32, 2, 2, true, 2>;
 using F32_128x128_k16 = F32Tile<128, 128, 16, 2, 2, false, 3, 1>;
+// Config 5's tile with direct-to-LDS operand loads: no staging registers,
+// 33 KB of LDS, four workgroups per CU (8192^3 140 -> 145, 16384^3 141.5 ->
+// 147, 4096^3 134 -> 141 TFLOP/s, same result bits).  The launcher runs it
+// for configs 5, 9 and 10 when the operand columns are 16-byte aligned.
+using F32_128x128_k16_dl = F32Tile<128, 128, 16, 2, 2, false, 4, 0, 1, 1, false, true>;
 // 64x64x64 with two k groups of four waves, loads two k-tiles ahead: for
 // grids of one 64x64 tile per CU (1024^3: 86 -> 92 TFLOP/s).
 using F32_64x64_kg2 = F32Tile<64, 64, 64, 2, 2, false, 1, 0, 2, 2>;
 // Short k (a few k-tiles): C prefetched at the start of the tile, so its
 // read overlaps the k loop.  (Loading two k-tiles ahead as well spills on
 // these tiles: 128x64 27, 128x128 84 VGPRs even without the C registers.)
+// The 128x64 tile takes 16-deep k-tiles: half the LDS (27 KB), four
+// workgroups per CU instead of three, so more tiles' C traffic is in flight
+// at once (8192x8192x127 95 -> 98, x64 67.5 -> 72 TFLOP/s).
 using F32_128x128_short = F32Tile<128, 128, 32, 2, 2, false, 2, 0, 1, 1, true>;
-using F32_128x64_short = F32Tile<128, 64, 32, 2, 2, false, 3, 0, 1, 1, true>;
+using F32_128x64_short = F32Tile<128, 64, 16, 2, 2, false, 4, 0, 1, 1, true>;
 
 template <class T>
 struct TypeTag {

@@ -382,7 +382,8 @@ TAIL_CONFIG = 9  # config 5's tile; the tiles of a partial last round split k
 
 
 @pytest.mark.parametrize("m,n,k,ld_pad", [
-    (3584, 3584, 1024, 0),   # 784 tiles of 128x128: 16 left over a 768-slot round
+    (3584, 3584, 1024, 0),   # 784 tiles of 128x128: one partial round of 1024 slots
+    (3584, 3584, 1024, 1),   # register-staged tile (ld not a multiple of 4): 768 + 16
     (3000, 4100, 777, 5),    # ragged edges and k chunks, padded leading dimensions
     (8192, 8064, 1000, 0),   # 4032 tiles: 192 left over
 ])
@@ -423,9 +424,10 @@ def test_tail_split_graph_replays_and_capture_fallback():
     """Captured after a warm-up call on the stream, the tail split replays
     bit-identically to its eager result.  Captured on a fresh stream (no
     workspace yet), it runs config 5's unsplit kernel, so every replay
-    equals config 5 bit for bit."""
+    equals config 5 bit for bit.  4096 x 4224: 1056 tiles of 128x128, one
+    full round of the direct-loaded tile's 1024 slots and 32 left over."""
     import torch
-    m = n = 3584
+    m, n = 4096, 4224
     k = 2048
     c = Case(m, n, k, seed=77, alpha=1.0, beta=0.0)
     dA, dB, dC = c.device()
