@@ -79,6 +79,8 @@ def parse():
                     help="form timed by --shape")
     ap.add_argument("--grouped-only", action="store_true",
                     help="run only the grouped leg (profiling runs) and exit")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="run only the serving leg and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -509,14 +511,25 @@ def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0, tune_iter
 def serving_leg(sa, torch, rates, window_s, chunk_gflop=0.0, seed=0):
     """Poisson arrivals of BF16x3 requests, sizes 70% [128,512], 25%
     [1024,2048], 5% [4096,8192] (each of M, N, K uniform in the bucket);
-    operands resident in HBM (sub-blocks of two 8192^2 panels), 64 GB budget."""
+    operands resident in HBM, 64 GB budget.  Each request reads its A and B
+    from a random place in 2 GiB pools (eight 8192^2 panels each: a random
+    panel, a random row offset that is a multiple of 4 and a random column
+    offset), so requests do not share operand lines in the caches."""
     import numpy as np
     rng = np.random.default_rng(seed)
     side = 8192
-    Apool = torch.rand(side * side, device="cuda") * 2 - 1
-    Bpool = torch.rand(side * side, device="cuda") * 2 - 1
+    panels = 8
+    Apool = torch.rand(panels * side * side, device="cuda") * 2 - 1
+    Bpool = torch.rand(panels * side * side, device="cuda") * 2 - 1
     arena = torch.zeros(2 << 30, device="cuda")   # 8 GiB ring for results
     ring = [0]
+
+    def operand(pool, rows, cols):
+        """Address of a rows x cols block (leading dimension `side`)."""
+        p = int(rng.integers(0, panels))
+        r0 = 4 * int(rng.integers(0, (side - rows) // 4 + 1))
+        c0 = int(rng.integers(0, side - cols + 1))
+        return pool.data_ptr() + 4 * (p * side * side + c0 * side + r0)
 
     buckets = ((0.70, 128, 512), (0.25, 1024, 2048), (0.05, 4096, 8192))
 
@@ -539,7 +552,8 @@ def serving_leg(sa, torch, rates, window_s, chunk_gflop=0.0, seed=0):
                     chunk_gflop=chunk_gflop)
 
     def submit(m, n, k):
-        return srv.submit(m, n, k, 1.0, Apool, side, Bpool, side, 0.0, c_slice(m, n), m)
+        return srv.submit(m, n, k, 1.0, operand(Apool, m, k), side, operand(Bpool, k, n), side,
+                          0.0, c_slice(m, n), m)
 
     # closed-loop capacity: a burst of requests, all queued at once.  Its
     # TFLOP/s is the capacity; in requests per second it is that over the
@@ -560,6 +574,7 @@ def serving_leg(sa, torch, rates, window_s, chunk_gflop=0.0, seed=0):
     out = {"capacity_req_s": capacity, "capacity_tflops": cap["tflops"],
            "capacity_req_s_burst": cap["completed"] / cap["window_s"],
            "mean_flops_per_request": mean_flops, "budget_gb": 64,
+           "operands": "random blocks of 2 x 2 GiB resident pools (8 panels of 8192^2 each)",
            "max_batch": 32, "slots": 3, "chunk_gflop": chunk_gflop, "rates": []}
     for frac in rates:
         lam = frac * capacity
@@ -679,6 +694,12 @@ def main():
             out[name] = grouped_leg(sa, torch, form, args.group_count, args.steps, args.warmup,
                                     stream, tune_iters=args.tune_iters)
         print(json.dumps({"grouped": out}), flush=True)
+        return
+    if args.serving_only:
+        sa.lib()
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        print(json.dumps({"serving": serving_leg(sa, torch, rates, args.serve_seconds,
+                                                 args.serve_chunk_gflop)}), flush=True)
         return
     sa.lib()
     torch.manual_seed(0)
