@@ -291,7 +291,9 @@ typedef struct {
 typedef struct {
     long long submitted, completed, rejected, failed;
     long long batches;
-    long long measured;   /* completed requests the latency figures cover */
+    long long measured;   /* completed requests since the last reset; mean, max
+                             and tflops cover all of them, p50 and p99 the
+                             most recent 65536 */
     double mean_batch;    /* requests per grouped launch */
     double p50_ms, p99_ms, mean_ms, max_ms; /* completion - submission */
     double mean_queue_ms;                    /* admission - submission */

@@ -155,6 +155,8 @@ struct Slot {
 constexpr double kDefaultChunkGflop = 100.0;
 // Finished requests whose status and latency stay queryable.
 constexpr size_t kHistory = 1 << 16;
+// Latencies kept for the percentiles: the most recent this many.
+constexpr size_t kLatencyWindow = 1 << 16;
 
 }  // namespace
 
@@ -292,12 +294,15 @@ class Server {
         return SGEMM_STATUS_SUCCESS;
     }
 
-    // Running figures since the last reset: the cost is one copy of the
-    // latencies measured since then, whatever the server's history.
+    // Running figures since the last reset.  Counts, means, the maximum and
+    // the throughput cover every request measured since then; p50 and p99
+    // the most recent kLatencyWindow of them, so the cost is bounded
+    // whatever the server's history.
     void stats(sgemm_server_stats* out) {
         std::memset(out, 0, sizeof(*out));
         std::vector<double> lat;
-        double qsum, first, last, flops;
+        double qsum, first, last, flops, lsum, lmax;
+        long long measured;
         {
             std::lock_guard<std::mutex> lock(mu_);
             out->submitted = submitted_;
@@ -314,8 +319,11 @@ class Server {
             first = first_submit_;
             last = last_done_;
             flops = flops_sum_;
+            lsum = lat_sum_;
+            lmax = lat_max_;
+            measured = measured_;
         }
-        out->measured = (long long)lat.size();
+        out->measured = measured;
         if (lat.empty()) return;
         std::sort(lat.begin(), lat.end());
         auto pct = [&](double p) {
@@ -325,11 +333,9 @@ class Server {
         };
         out->p50_ms = pct(50);
         out->p99_ms = pct(99);
-        out->max_ms = lat.back();
-        double s = 0;
-        for (double v : lat) s += v;
-        out->mean_ms = s / (double)lat.size();
-        out->mean_queue_ms = qsum / (double)lat.size();
+        out->max_ms = lmax;
+        out->mean_ms = lsum / (double)measured;
+        out->mean_queue_ms = qsum / (double)measured;
         out->window_s = last - first;
         out->tflops = last > first ? flops / (last - first) / 1e12 : 0.0;
     }
@@ -342,6 +348,10 @@ class Server {
         batched_requests_ = 0;
         panel_launches_ = 0;
         lat_ms_.clear();
+        lat_next_ = 0;
+        measured_ = 0;
+        lat_sum_ = 0;
+        lat_max_ = 0;
         queue_ms_sum_ = 0;
         flops_sum_ = 0;
         first_submit_ = 1e300;
@@ -455,7 +465,13 @@ class Server {
         if (q->status == SGEMM_STATUS_SUCCESS) {
             ++completed_;
             if (q->t_submit >= stats_epoch_) {
-                lat_ms_.push_back((q->t_done - q->t_submit) * 1e3);
+                const double ms = (q->t_done - q->t_submit) * 1e3;
+                if (lat_ms_.size() < kLatencyWindow) lat_ms_.push_back(ms);
+                else lat_ms_[lat_next_] = ms;
+                lat_next_ = (lat_next_ + 1) % kLatencyWindow;
+                ++measured_;
+                lat_sum_ += ms;
+                lat_max_ = std::max(lat_max_, ms);
                 queue_ms_sum_ += (q->t_admit - q->t_submit) * 1e3;
                 flops_sum_ += 2.0 * q->r.m * q->r.n * (double)q->r.k;
                 first_submit_ = std::min(first_submit_, q->t_submit);
@@ -708,8 +724,13 @@ class Server {
     long long batches_ = 0, batched_requests_ = 0;
     size_t used_bytes_ = 0, peak_bytes_ = 0;
     double stats_epoch_ = 0;
-    // Running statistics of the requests submitted since stats_epoch_.
+    // Running statistics of the requests submitted since stats_epoch_:
+    // the latency ring (the last kLatencyWindow, lat_next_ the slot written
+    // next once it is full) and sums over all of them.
     std::vector<double> lat_ms_;
+    size_t lat_next_ = 0;
+    long long measured_ = 0;
+    double lat_sum_ = 0, lat_max_ = 0;
     double queue_ms_sum_ = 0, flops_sum_ = 0, first_submit_ = 1e300, last_done_ = 0;
 };
 

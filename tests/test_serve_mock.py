@@ -240,7 +240,9 @@ def test_batch_failure_fails_only_its_requests():
 
 def test_history_is_bounded_and_stats_stay_complete():
     """Finished requests leave the history past 65536 (their ids return
-    NOT_FOUND); the running statistics still cover every request."""
+    NOT_FOUND); the running statistics still cover every request (counts,
+    mean and max over all of them, the percentiles over the most recent
+    65536 latencies)."""
     srv = sa.Server(1 << 30, max_batch=256, num_slots=4, executor=sa.EXECUTOR_MOCK,
                     mock_tflops=1e6, mock_launch_us=0)
     count = 65536 + 1000
@@ -251,6 +253,9 @@ def test_history_is_bounded_and_stats_stay_complete():
     assert codes[0] == 7 and codes[-1] == 0
     st = srv.stats()
     assert st["completed"] == count and st["measured"] == count
+    kept = [srv.latency(rid)[0] for rid in ids[-65536:]]
+    assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    assert st["max_ms"] >= max(kept) and 0 < st["mean_ms"] <= st["max_ms"]
     srv.reset_stats()
     assert srv.stats()["measured"] == 0
     srv.destroy()
