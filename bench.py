@@ -116,7 +116,7 @@ def sweep(sa, torch, warmup):
             ms = time_form(lambda: fn(m, n, k, 1.5, A, m, B, k, 0.5, C, m, stream), it, warm,
                            stream)
             tf = flops / (ms * 1e-3) / 1e12
-            cfg = sa.autotune_lookup(form, m, n, k)[0]
+            cfg = sa.launch_config(form, m, n, k, A, m, B, k)
             row[name] = {"ms": ms, "tflops": tf, "frac_of_peak": tf / peak, "config": cfg}
         print(f"sweep {m}x{n}x{k}: f32 {row['f32']['tflops']:.1f} TF, "
               f"bf16x3 {row['bf16x3']['tflops']:.1f} TF", file=sys.stderr, flush=True)
@@ -152,7 +152,7 @@ def sweep_compact(sa, torch, stream, target_s=0.08):
             ms = time_form(call, it, max(2, it // 2), stream)
             tf = flops / (ms * 1e-3) / 1e12
             row[name] = {"tflops": tf, "frac_of_peak": tf / peak, "ms": ms, "calls": it,
-                         "config": sa.autotune_lookup(form, m, n, k)[0]}
+                         "config": sa.launch_config(form, m, n, k, A, m, B, k)}
         rows.append(row)
         del A, B, C
     torch.cuda.empty_cache()
@@ -684,7 +684,7 @@ def main():
         ms = time_form(call, args.steps, 0, stream)
         print(json.dumps({"shape": [m, n, k], "form": args.form, "ms": ms,
                           "tflops": 2.0 * m * n * k / (ms * 1e-3) / 1e12,
-                          "config": sa.autotune_lookup(form, m, n, k)[0]}), flush=True)
+                          "config": sa.launch_config(form, m, n, k, A, m, B, k)}), flush=True)
         return
     if args.grouped_only:
         sa.lib()

@@ -156,6 +156,14 @@ int sgemm_run_config(int form, int config_id, int m, int n, int k, const float* 
                      const float* A, int lda, const float* B, int ldb, const float* beta,
                      float* C, int ldc, sgemm_stream_t stream);
 int sgemm_config_count(int form);
+/* Config id that sgemm_ex (form SGEMM_FORM_F32) or sgemm_bf16x3 would launch
+ * for this problem with these operands (the table or heuristic choice, which
+ * depends on the shape and, for FP32, on whether A, B, lda and ldb give
+ * 16-byte aligned columns); -1 where no tile config runs (k == 0, an empty
+ * problem, or a leading dimension past 2^21, which take the scale-only or
+ * the naive kernel).  Reads no memory; no reference counterpart. */
+int sgemm_launch_config(int form, int m, int n, int k, const float* A, int lda, const float* B,
+                        int ldb, int* config_id);
 /* Writes "BMxBNxBK/WMxWN" style text; returns the tile sizes. */
 int sgemm_config_info(int form, int config_id, int* bm, int* bn, int* bk, char* name,
                       int name_len);

@@ -190,6 +190,19 @@ int sgemm_run_config(int form, int config_id, int m, int n, int k, const float* 
 
 int sgemm_config_count(int form) { return config_count(form); }
 
+int sgemm_launch_config(int form, int m, int n, int k, const float* A, int lda, const float* B,
+                        int ldb, int* config_id) {
+    if (form != SGEMM_FORM_F32 && form != SGEMM_FORM_BF16X3) return SGEMM_STATUS_NOT_SUPPORTED;
+    if (config_id == nullptr || m < 0 || n < 0 || k < 0) return SGEMM_STATUS_INVALID_VALUE;
+    *config_id = -1;
+    // As run_problem: nothing to multiply, or leading dimensions past the
+    // tiles' 32-bit offsets, launch no tile config.
+    if (m == 0 || n == 0 || k == 0 || lda > kMaxLd || ldb > kMaxLd) return SGEMM_STATUS_SUCCESS;
+    const Problem p = make_problem(m, n, k, 1.f, A, lda, B, ldb, 0.f, nullptr, m);
+    *config_id = choose_config(form, m, n, k, columns_16b_aligned(p));
+    return SGEMM_STATUS_SUCCESS;
+}
+
 int sgemm_config_info(int form, int config_id, int* bm, int* bn, int* bk, char* name,
                       int name_len) {
     if (config_id < 0 || config_id >= config_count(form)) return SGEMM_STATUS_NOT_SUPPORTED;

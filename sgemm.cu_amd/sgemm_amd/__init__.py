@@ -164,6 +164,7 @@ def lib() -> ctypes.CDLL:
         "sgemm_autotune_bucket": [i, i, i, i, i, i, vp, ctypes.POINTER(i),
                                   ctypes.POINTER(ctypes.c_float)],
         "sgemm_autotune_lookup": [i, i, i, i, i, ctypes.POINTER(i), ctypes.POINTER(i)],
+        "sgemm_launch_config": [i, i, i, i, vp, i, vp, i, ctypes.POINTER(i)],
         "sgemm_autotune_set": [i, i, i, i, i, i],
         "sgemm_autotune_clear": [],
         "sgemm_autotune_size": [],
@@ -389,6 +390,14 @@ def autotune_lookup(form, m, n, k, grouped=False):
     _check(lib().sgemm_autotune_lookup(form, int(grouped), m, n, k, ctypes.byref(cfg),
                                        ctypes.byref(from_table)), "sgemm_autotune_lookup")
     return cfg.value, bool(from_table.value)
+
+
+def launch_config(form, m, n, k, A, lda, B, ldb) -> int:
+    """Config the default entry point launches for these operands (-1: none)."""
+    cfg = ctypes.c_int()
+    _check(lib().sgemm_launch_config(form, m, n, k, _ptr(A), lda, _ptr(B), ldb,
+                                     ctypes.byref(cfg)), "sgemm_launch_config")
+    return cfg.value
 
 
 def autotune_set(form, m, n, k, config_id, grouped=False):

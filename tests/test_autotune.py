@@ -86,3 +86,30 @@ def test_environment_table_loaded_on_first_use():
         out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
                              text=True, check=True).stdout.strip()
         assert out == "(2, True)"
+
+
+def test_launch_config_follows_operand_alignment():
+    """sgemm_launch_config reports what the default entry point launches
+    (pointers are not read).  FP32 with 16-byte aligned columns takes the
+    direct-to-LDS tile, 1024 resident, so 700-1023 tiles of 128x128 with
+    k >= 1024 go to stream-K (config 11); with odd leading dimensions the
+    register-staged tile (768 resident) takes the tail split (config 9)
+    from 700 tiles.  k == 0 and leading dimensions past 2^21 run no tile
+    config (-1)."""
+    F = sa.FORM_F32
+    base = 1 << 20                                # 16-byte aligned fake addresses
+    m = n = 3584                                  # 784 tiles of 128x128
+    k = 3584
+    aligned = sa.launch_config(F, m, n, k, base, m, base, k)
+    assert aligned == sa.autotune_lookup(F, m, n, k)[0] == 11
+    assert sa.launch_config(F, m, n, k, base, m + 1, base, k) == 9      # odd lda
+    assert sa.launch_config(F, m, n, k, base + 4, m, base, k) == 9      # A 4 bytes off
+    assert sa.launch_config(F, 8192, 8192, 8192, base, 8192, base, 8192) == 9
+    assert sa.launch_config(F, 8192, 8192, 127, base, 8193, base, 127) == 14
+    assert sa.launch_config(F, m, n, 0, base, m, base, 1) == -1
+    assert sa.launch_config(F, m, n, k, base, (1 << 21) + 4, base, k) == -1
+    for form in (sa.FORM_F32, sa.FORM_BF16X3):                          # table entries win
+        sa.autotune_set(form, m, n, k, 2)
+        assert sa.launch_config(form, m, n, k, base, m + 1, base, k) == 2
+    with pytest.raises(sa.SgemmError):
+        sa.launch_config(F, -1, n, k, base, m, base, k)
