@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--no-grouped", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the compact size sweep")
     ap.add_argument("--no-vendor", action="store_true", help="skip the vendor BLAS line")
-    ap.add_argument("--tune-iters", type=int, default=10,
+    ap.add_argument("--tune-iters", type=int, default=20,
                     help="timed launches per candidate when the grouped leg tunes its buckets")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--group-count", type=int, default=512)
@@ -285,6 +285,14 @@ def time_form(fn, steps, warmup, stream):
     return start.elapsed_time(stop) / steps  # ms per call
 
 
+def settle(fn, stream, warmup, seconds=0.3):
+    """At least `warmup` calls and about `seconds` of back-to-back calls, so
+    the engine clock has settled before a short launch is timed (the first
+    few milliseconds after a phase of small launches run slower)."""
+    probe = time_form(fn, 2, 1, stream)
+    time_form(fn, max(warmup, int(seconds * 1e3 / max(probe, 1e-3))), 0, stream)
+
+
 def per_call(fn, calls, stream, flush=None):
     """Each call timed on its own (one HIP event pair per call): the median,
     and the mean of the second half as the reference's harness reports it
@@ -449,7 +457,7 @@ def cpu_baseline(size: int, cols: int):
             "host_cpus_visible": os.cpu_count(), "cpu_model": model}
 
 
-def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0, tune_iters=10):
+def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0, tune_iters=20):
     """512 independent problems, M, N, K log-uniform integers in [32, 2048],
     U[-1, 1), alpha=1, beta=0: one persistent launch replayed from a hipGraph,
     against the same problems as back-to-back single launches of the default
@@ -468,7 +476,14 @@ def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0, tune_iter
         mats.append((A, B, C))
         probs.append(sa.problem(m, n, k, 1.0, A, m, B, k, 0.0, C, m))
         flops += 2.0 * m * n * k
-    # autotune every size bucket present (grouped-capable configs only)
+    # autotune every size bucket present (grouped-capable configs only),
+    # after about 0.3 s of 2048^3 launches so that the first candidates are
+    # not timed on a clock that is still rising
+    single = sa.sgemm_ex if form == sa.FORM_F32 else sa.sgemm_bf16x3
+    W = torch.rand(2048 * 2048, device="cuda")
+    settle(lambda: single(2048, 2048, 2048, 1.0, W, 2048, W, 2048, 0.0, W, 2048, stream), stream,
+           warmup)
+    del W
     t0 = time.perf_counter()
     tuned = {}
     for m, n, k in shapes.tolist():
@@ -482,11 +497,11 @@ def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0, tune_iter
     plan = sa.GroupedPlan(count, form)
     plan.set(probs, gs)
     plan.capture(gs)
-    ms_group = time_form(lambda: plan.replay(gs), steps, warmup, gs)
+    settle(lambda: plan.replay(gs), gs, warmup)
+    ms_group = time_form(lambda: plan.replay(gs), steps, 0, gs)
     # the same problems as single launches of the library's default entry
     # point (each with its own single-launch config), captured in one graph on
     # a stream warmed up first (so split-K workspaces exist before capture)
-    single = sa.sgemm_ex if form == sa.FORM_F32 else sa.sgemm_bf16x3
     loop_graph = torch.cuda.CUDAGraph()
     side = torch.cuda.Stream()
     side.wait_stream(stream)
@@ -497,7 +512,8 @@ def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0, tune_iter
     with torch.cuda.graph(loop_graph, stream=side):
         for (m, n, k), (A, B, C) in zip(shapes.tolist(), mats):
             single(m, n, k, 1.0, A, m, B, k, 0.0, C, m, side)
-    ms_loop = time_form(loop_graph.replay, steps, warmup, stream)
+    settle(loop_graph.replay, stream, warmup)
+    ms_loop = time_form(loop_graph.replay, steps, 0, stream)
     info = plan.info()
     plan.destroy()
     return {"problems": count, "flops": flops, "buckets_tuned": len(tuned),
@@ -679,8 +695,7 @@ def main():
             fn(m, n, k, 1.5, A, m, B, k, 0.5, C, m, stream)
         # Warm up for about 0.3 s so the engine clock has settled (a few
         # calls of a short problem end before it rises), then time `steps`.
-        probe = time_form(call, 3, 1, stream)
-        time_form(call, max(args.warmup, int(300.0 / max(probe, 1e-3))), 0, stream)
+        settle(call, stream, args.warmup)
         ms = time_form(call, args.steps, 0, stream)
         print(json.dumps({"shape": [m, n, k], "form": args.form, "ms": ms,
                           "tflops": 2.0 * m * n * k / (ms * 1e-3) / 1e12,
