@@ -222,7 +222,12 @@ hipError_t launch_f32(int id, const Problem& p, hipStream_t s, const Arena* aren
             return dl ? launch_tile<F32_128x128_k16_dl, false>(p, s, arena)
                       : launch_tile<F32_128x128_k16, false>(p, s, arena);
         case 6: return launch_tile<F32_128x128, true>(p, s, arena);
-        case 7: return launch_tile<F32_64x64, true>(p, s, arena);
+        case 1:
+            return dl ? launch_tile<F32_64x64_dl, false>(p, s, arena)
+                      : launch_tile<F32_64x64, false>(p, s, arena);
+        case 7:
+            return dl ? launch_tile<F32_64x64_dl, true>(p, s, arena)
+                      : launch_tile<F32_64x64, true>(p, s, arena);
         case 8: return launch_tile<F32_128x64, true>(p, s, arena);
         case 9:
             return dl ? launch_tail_split<F32_128x128_k16_dl>(p, s, arena)

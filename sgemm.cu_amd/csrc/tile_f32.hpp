@@ -57,14 +57,16 @@ namespace sgemm_amd {
 // 16 * lane), so no registers stage them and no LDS stores follow the loads;
 // the wave waits for its own loads before the k-tile's closing barrier.  The
 // placement is lane-contiguous, so the layout is chosen through which HBM
-// address each lane reads: A as pairs of k rows (2*BM floats, m-contiguous)
-// with 8 floats of padding after each pair, B as rows of BK k-values per
-// column with the 16-byte chunks of row n rotated by XOR with (n / 4) % 4.
-// Both make the fragment reads bank-conflict free.  A k-tile that reaches
-// past K goes through registers (masked) into the same layout, so the
-// arithmetic and the result bits are those of the register-staged tile.
-// Needs 16-byte aligned operand columns (A, B 16-byte aligned, lda and ldb
-// multiples of 4); the launcher picks the register-staged tile otherwise.
+// address each lane reads.  A: blocks of 256 / BM k rows (one instruction,
+// m-contiguous) with 64 / BM * 4 floats of padding after each block, so the
+// four k rows 4 apart that one MFMA step reads start 16 banks apart.  B: rows
+// of BK k-values per column with the 16-byte chunks of row n rotated by XOR
+// with (n / 4) % 4 (BK 16) or (n / 2) % 8 (BK 32), so the 16 columns of a
+// fragment read fall on distinct banks.  A k-tile that reaches past K goes
+// through registers (masked) into the same layout, so the arithmetic and the
+// result bits are those of the register-staged tile.  Needs 16-byte aligned
+// operand columns (A, B 16-byte aligned, lda and ldb multiples of 4); the
+// launcher picks the register-staged tile otherwise.
 template <int BM_, int BN_, int BK_, int WM_, int WN_, bool AT_ = false, int WPE_ = 1,
           int SCHED_ = 0, int PF_ = 1, int KG_ = 1, bool CPF_ = false, bool DL_ = false>
 struct F32Tile {
@@ -80,10 +82,12 @@ struct F32Tile {
     static constexpr int TM = BM / WM, TN = BN / WN;  // wave tile
     static constexpr int MB = TM / 16, NB = TN / 16;  // 16x16 MFMA blocks per wave
     // A layout: As[k][m] with row stride BM+4, As[m][k] with row stride BK+4
-    // (AT), or pairs of k rows with stride APAIR (DL).
+    // (AT), or blocks of ARB k rows with stride ABLK (DL).
     static constexpr int SA = AT ? BK + 4 : BM + 4;
-    static constexpr int APAIR = 2 * BM + 8;
-    static constexpr int A_FLOATS = DL ? (BK / 2) * APAIR : AT ? BM * SA : BK * SA;
+    static constexpr int ARB = 256 / BM;               // DL: k rows per A block
+    static constexpr int ABLK = 256 + 4 * ARB;         // DL: A block stride (floats)
+    static constexpr int BRB = 256 / BK;               // DL: B columns per block
+    static constexpr int A_FLOATS = DL ? (BK / ARB) * ABLK : AT ? BM * SA : BK * SA;
     static constexpr int SB = DL ? BK : BK + 4;       // Bs[n][k] row stride (floats)
     static constexpr int STAGE = A_FLOATS + BN * SB;  // floats per LDS stage
     static constexpr int LDS_FLOATS = 2 * STAGE;
@@ -104,12 +108,12 @@ struct F32Tile {
                   "k-group sums fit in LDS");
     static_assert(!CPF || KG == 1, "C prefetch on one-group tiles");
     static_assert(epilogue_lds_floats(WM * WN, TM, 16) <= LDS_FLOATS, "epilogue LDS");
-    // DL: one load instruction fills one pair of A rows (BM = 128: 64 lanes x
-    // 4 floats x 2 rows) or 16 B rows of 16 k (BK = 16); each wave issues
-    // A_VEC of the first kind and B_VEC of the second per k-tile.
-    static_assert(!DL || (BM == 128 && BK == 16 && !AT && PF == 1 && KG == 1 && !CPF &&
-                          SCHED == 0 && A_VEC * WM * WN == BK / 2 &&
-                          B_VEC * WM * WN == BN / 16),
+    // DL: one load instruction fills one A block (ARB k rows of BM) or one B
+    // block (BRB columns of BK); each wave issues A_VEC of the first kind and
+    // B_VEC of the second per k-tile.
+    static_assert(!DL || ((BM == 64 || BM == 128) && (BK == 16 || BK == 32) && !AT &&
+                          PF == 1 && KG == 1 && !CPF && SCHED == 0 &&
+                          A_VEC * WM * WN == BK / ARB && B_VEC * WM * WN == BN / BRB),
                   "direct-to-LDS tile shape");
 };
 
@@ -204,18 +208,20 @@ struct F32TileRunner {
         const int b_k = 4 * (t % S);
         const int b_q = t / 16;
         const int b_n = (b_q % S) + S * ((t / S) % R) + 16 * (b_q / S);
-        // DL: load instruction r of this wave fills LDS block
-        // wave * A_VEC + r of A (wave * B_VEC + r of B).  A: lanes 0-31 read k row 2 * block, lanes 32-63 row
-        // 2 * block + 1, four m each.  B: lane / 4 is the column within the
-        // block's 16, lane % 4 the chunk position, which holds k-chunk
-        // (lane % 4) ^ ((lane / 16) % 4).
+        // DL: load instruction r of this wave fills LDS block wave * A_VEC + r
+        // of A (wave * B_VEC + r of B).  A: lane / (BM / 4) is the k row
+        // within the block, lane % (BM / 4) the four m it reads.  B: lane /
+        // (BK / 4) is the column within the block, lane % (BK / 4) the chunk
+        // position, which holds k-chunk (lane % (BK / 4)) ^ dl_swz(column).
         const int dl_wave = __builtin_amdgcn_readfirstlane(wave);
-        const int dl_row = lane >> 2, dl_kc = (lane & 3) ^ ((dl_row >> 2) & 3);
+        const int dl_arow = lane / (BM / 4), dl_am = 4 * (lane % (BM / 4));
+        const int dl_col = lane / (BK / 4);
+        auto dl_swz = [](int n) { return BK == 16 ? (n >> 2) & 3 : (n >> 1) & 7; };
         int a_off[T::A_VEC], b_off[T::B_VEC];
 #pragma unroll
         for (int r = 0; r < T::A_VEC; ++r) {
             if constexpr (T::DL)
-                a_off[r] = 4 * (4 * (lane & 31) + (2 * (dl_wave * T::A_VEC + r) + (lane >> 5)) * P.lda);
+                a_off[r] = 4 * (dl_am + (T::ARB * (dl_wave * T::A_VEC + r) + dl_arow) * P.lda);
             else if constexpr (T::AT)
                 a_off[r] = 4 * (4 * (at_m4 + (r / 4) * (T::NT / AS)) +
                                 (4 * at_k4 + r % 4) * P.lda);
@@ -224,8 +230,10 @@ struct F32TileRunner {
         }
 #pragma unroll
         for (int r = 0; r < T::B_VEC; ++r) {
-            if constexpr (T::DL)
-                b_off[r] = 4 * (4 * dl_kc + (16 * (dl_wave * T::B_VEC + r) + dl_row) * P.ldb);
+            if constexpr (T::DL) {
+                const int n = T::BRB * (dl_wave * T::B_VEC + r) + dl_col;
+                b_off[r] = 4 * (4 * ((lane % (BK / 4)) ^ dl_swz(n)) + n * P.ldb);
+            }
             else
                 b_off[r] = 4 * (b_k + (b_n + r * T::B_NSTEP) * P.ldb);
         }
@@ -256,7 +264,7 @@ struct F32TileRunner {
             const auto br = make_rsrc(P.B + b_base, 4 * (b_ext - b_base));
 #pragma unroll
             for (int r = 0; r < T::A_VEC; ++r)
-                buffer_load_lds16(ar, stage + (dl_wave * T::A_VEC + r) * T::APAIR, a_off[r]);
+                buffer_load_lds16(ar, stage + (dl_wave * T::A_VEC + r) * T::ABLK, a_off[r]);
 #pragma unroll
             for (int r = 0; r < T::B_VEC; ++r)
                 buffer_load_lds16(br, stage + T::A_FLOATS + (dl_wave * T::B_VEC + r) * 256, b_off[r]);
@@ -271,13 +279,15 @@ struct F32TileRunner {
                 // The slots dma() would fill, k >= K zeroed.
 #pragma unroll
                 for (int r = 0; r < T::A_VEC; ++r) {
-                    const bool ok = !MASK || k0 + 2 * (dl_wave * T::A_VEC + r) + (lane >> 5) < K;
-                    *reinterpret_cast<f32x4*>(&As[(dl_wave * T::A_VEC + r) * T::APAIR + 4 * lane]) =
+                    const bool ok =
+                        !MASK || k0 + T::ARB * (dl_wave * T::A_VEC + r) + dl_arow < K;
+                    *reinterpret_cast<f32x4*>(&As[(dl_wave * T::A_VEC + r) * T::ABLK + 4 * lane]) =
                         ok ? ra[q][r] : f32x4{0.f, 0.f, 0.f, 0.f};
                 }
-                const int kv = K - (k0 + 4 * dl_kc);
 #pragma unroll
                 for (int r = 0; r < T::B_VEC; ++r) {
+                    const int n = T::BRB * (dl_wave * T::B_VEC + r) + dl_col;
+                    const int kv = K - (k0 + 4 * ((lane % (BK / 4)) ^ dl_swz(n)));
                     f32x4 v;
 #pragma unroll
                     for (int e = 0; e < 4; ++e) v[e] = !MASK || e < kv ? rb[q][r][e] : 0.f;
@@ -402,8 +412,9 @@ struct F32TileRunner {
                     if constexpr (T::DL) {
 #pragma unroll
                         for (int s = 0; s < 4; ++s)
-                            a[i][s] = As[(kk / 2 + 2 * g + s / 2) * T::APAIR + (s % 2) * BM +
-                                         wm * T::TM + 16 * i + l16];
+                            a[i][s] = As[((kk + 4 * g + s) / T::ARB) * T::ABLK +
+                                         ((kk + 4 * g + s) % T::ARB) * BM + wm * T::TM +
+                                         16 * i + l16];
                     } else if constexpr (T::AT) {
                         a[i] = *reinterpret_cast<const f32x4*>(
                             &As[(wm * T::TM + 16 * i + l16) * T::SA + kk + 4 * g]);
@@ -415,9 +426,11 @@ struct F32TileRunner {
                 }
 #pragma unroll
                 for (int j = 0; j < T::NB; ++j) {
-                    if constexpr (T::DL)
+                    if constexpr (T::DL) {
+                        const int n = wn * T::TN + 16 * j + l16;
                         b[j] = *reinterpret_cast<const f32x4*>(
-                            &Bs[(wn * T::TN + 16 * j + l16) * T::SB + 4 * (g ^ ((l16 >> 2) & 3))]);
+                            &Bs[n * T::SB + 4 * ((kk / 4 + g) ^ dl_swz(n))]);
+                    }
                     else
                         b[j] = *reinterpret_cast<const f32x4*>(
                             &Bs[(wn * T::TN + 16 * j + l16) * T::SB + kk + 4 * g]);
@@ -581,6 +594,10 @@ using F32_128x128_k16 = F32Tile<128, 128, 16, 2, 2, false, 3, 1>;
 // 147, 4096^3 134 -> 141 TFLOP/s, same result bits).  The launcher runs it
 // for configs 5, 9 and 10 when the operand columns are 16-byte aligned.
 using F32_128x128_k16_dl = F32Tile<128, 128, 16, 2, 2, false, 4, 0, 1, 1, false, true>;
+// Config 1/7's 64x64x32 tile with direct-to-LDS loads (34 KB of LDS, four
+// workgroups per CU; same result bits as the register-staged tile, which
+// loads two k-tiles ahead): 2048^3 121 -> 129, 3072^3 127.5 -> 133 TFLOP/s.
+using F32_64x64_dl = F32Tile<64, 64, 32, 2, 2, false, 4, 0, 1, 1, false, true>;
 // 64x64x64 with two k groups of four waves, loads two k-tiles ahead: for
 // grids of one 64x64 tile per CU (1024^3: 86 -> 92 TFLOP/s).
 using F32_64x64_kg2 = F32Tile<64, 64, 64, 2, 2, false, 1, 0, 2, 2>;

@@ -1,12 +1,14 @@
-"""Config 5's tile with direct-to-LDS operand loads (configs 5, 9 and 10 run
-it when A, B, lda and ldb give 16-byte aligned columns; otherwise the
-register-staged tile runs).  Both tiles do the same arithmetic in the same
+"""The FP32 tiles with direct-to-LDS operand loads: config 5's 128x128x16
+tile (configs 5, 9 and 10) and config 1/7's 64x64x32 tile run them when A, B,
+lda and ldb give 16-byte aligned columns; otherwise the register-staged tile
+runs.  Both tiles do the same arithmetic in the same
 order, so the same matrices give the same bits whichever one runs: each case
 here runs the aligned layout (direct loads) and the same values at ld + 1 or
 at a 4-byte offset (register staging), compares the bits, and checks the
 aligned result against the FP64 oracle.  The k values cover one partial
-k-tile (k < 16, register path only), exactly one and two full k-tiles, a
-partial last k-tile after the direct-loaded ones, and long k."""
+k-tile (k below the tile's 16 or 32, register path only), exactly one and two
+full k-tiles, a partial last k-tile after the direct-loaded ones, and long
+k."""
 import numpy as np
 import pytest
 
@@ -17,7 +19,8 @@ from gpu_util import Case, assert_ok, check, compact, download, log, upload
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(128, 128, 16), (256, 128, 32), (130, 257, 33), (300, 257, 129), (129, 130, 1000),
-          (64, 64, 8192), (1000, 1000, 1000), (127, 385, 7), (513, 260, 47)]
+          (64, 64, 8192), (1000, 1000, 1000), (127, 385, 7), (513, 260, 47), (64, 192, 64),
+          (200, 72, 95)]
 
 
 def _aligned(v):
@@ -47,7 +50,7 @@ def _run(config, c, A, lda, B, ldb, a_off=0, b_off=0):
     return out
 
 
-@pytest.mark.parametrize("config", [5, 9, 10])
+@pytest.mark.parametrize("config", [1, 5, 7, 9, 10])
 def test_direct_loads_match_register_staging(config):
     for (m, n, k) in SHAPES:
         lda, ldb = _aligned(m), _aligned(k)
@@ -68,7 +71,7 @@ def test_direct_loads_match_register_staging(config):
             f"offset pointers {(m, n, k)} config {config}"
 
 
-@pytest.mark.parametrize("config", [5, 9, 10])
+@pytest.mark.parametrize("config", [1, 5, 7, 9, 10])
 def test_direct_loads_integer_kat_bit_exact(config):
     """Integer inputs in [-4, 4] with aligned columns: exact in any order."""
     for (m, n, k) in [(300, 256, 129), (132, 128, 1000), (64, 64, 8192), (128, 128, 16)]:
@@ -80,11 +83,11 @@ def test_direct_loads_integer_kat_bit_exact(config):
 
 
 def test_direct_loads_tail_split_and_splitk_at_scale():
-    """Configs 9 (tail split) and 10 (split-K) with the direct-loaded tile on
-    grids that split: bit-identical to config 5 on integer data, and config 5
-    equal to the FP64 oracle on sampled entries (integer data is exact)."""
+    """Configs 9 (tail split), 10 and 7 (split-K) with the direct-loaded tiles
+    on grids that split: bit-identical to config 5 on integer data, and config
+    5 equal to the FP64 oracle on sampled entries (integer data is exact)."""
     for (m, n, k, config) in [(3584, 3584, 1024, 9), (2048, 1024, 4096, 10),
-                              (8192, 8064, 1008, 9)]:
+                              (8192, 8064, 1008, 9), (1024, 512, 8192, 7)]:
         c = Case(m, n, k, kind="int", seed=m + k, alpha=1.5, beta=0.5)
         split = _run(config, c, c.A, c.lda, c.B, c.ldb)
         whole = _run(5, c, c.A, c.lda, c.B, c.ldb)
