@@ -58,8 +58,9 @@ namespace sgemm_amd {
 // the wave waits for its own loads before the k-tile's closing barrier.  The
 // placement is lane-contiguous, so the layout is chosen through which HBM
 // address each lane reads.  A: blocks of 256 / BM k rows (one instruction,
-// m-contiguous) with 64 / BM * 4 floats of padding after each block, so the
-// four k rows 4 apart that one MFMA step reads start 16 banks apart.  B: rows
+// m-contiguous) with 1024 / BM floats of padding after each block (8 for BM
+// 128, 16 for 64), so the four k rows 4 apart that one MFMA step reads start
+// 16 banks apart.  B: rows
 // of BK k-values per column with the 16-byte chunks of row n rotated by XOR
 // with (n / 4) % 4 (BK 16) or (n / 2) % 8 (BK 32), so the 16 columns of a
 // fragment read fall on distinct banks.  A k-tile that reaches past K goes
