@@ -61,6 +61,8 @@ def parse():
                     help="arrival window per swept rate")
     ap.add_argument("--serve-chunk-gflop", type=float, default=0.0,
                     help="server chunk threshold in GFLOP (0 = library default, < 0 = off)")
+    ap.add_argument("--serve-slots", type=int, default=3,
+                    help="server slots (streams with a launch in flight)")
     ap.add_argument("--serve-rates", type=str, default="0.25,0.5,0.75,0.9,1.0",
                     help="arrival rates as fractions of the measured closed-loop capacity "
                          "(at 1.0 the queue does not settle: its p99 grows with the window)")
@@ -524,7 +526,7 @@ def grouped_leg(sa, torch, form, count, steps, warmup, stream, seed=0, tune_iter
             "grid": info["grid"]}
 
 
-def serving_leg(sa, torch, rates, window_s, chunk_gflop=0.0, seed=0):
+def serving_leg(sa, torch, rates, window_s, chunk_gflop=0.0, seed=0, slots=3):
     """Poisson arrivals of BF16x3 requests, sizes 70% [128,512], 25%
     [1024,2048], 5% [4096,8192] (each of M, N, K uniform in the bucket);
     operands resident in HBM, 64 GB budget.  Each request reads its A and B
@@ -564,7 +566,7 @@ def serving_leg(sa, torch, rates, window_s, chunk_gflop=0.0, seed=0):
         ring[0] += (m * n + 63) // 64 * 64
         return p
 
-    srv = sa.Server(64 << 30, max_batch=32, num_slots=3, form=sa.FORM_BF16X3,
+    srv = sa.Server(64 << 30, max_batch=32, num_slots=slots, form=sa.FORM_BF16X3,
                     chunk_gflop=chunk_gflop)
 
     def submit(m, n, k):
@@ -591,7 +593,7 @@ def serving_leg(sa, torch, rates, window_s, chunk_gflop=0.0, seed=0):
            "capacity_req_s_burst": cap["completed"] / cap["window_s"],
            "mean_flops_per_request": mean_flops, "budget_gb": 64,
            "operands": "random blocks of 2 x 2 GiB resident pools (8 panels of 8192^2 each)",
-           "max_batch": 32, "slots": 3, "chunk_gflop": chunk_gflop, "rates": []}
+           "max_batch": 32, "slots": slots, "chunk_gflop": chunk_gflop, "rates": []}
     for frac in rates:
         lam = frac * capacity
         n_req = max(50, int(lam * window_s))
@@ -714,7 +716,8 @@ def main():
         sa.lib()
         rates = [float(r) for r in args.serve_rates.split(",") if r]
         print(json.dumps({"serving": serving_leg(sa, torch, rates, args.serve_seconds,
-                                                 args.serve_chunk_gflop)}), flush=True)
+                                                 args.serve_chunk_gflop,
+                                                 slots=args.serve_slots)}), flush=True)
         return
     sa.lib()
     torch.manual_seed(0)
@@ -798,7 +801,8 @@ def main():
     if not args.no_serve:
         print("bench: serving", file=sys.stderr, flush=True)
         rates = [float(x) for x in args.serve_rates.split(",") if x]
-        serving = serving_leg(sa, torch, rates, args.serve_seconds, args.serve_chunk_gflop)
+        serving = serving_leg(sa, torch, rates, args.serve_seconds, args.serve_chunk_gflop,
+                              slots=args.serve_slots)
 
     achieved = tflops
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
