@@ -89,9 +89,13 @@ int heuristic(int form, int grouped, int m, int n, int k, bool aligned) {
         // Short k (<= 256) is bound by reading and writing C: 128x64 with
         // C prefetched at the start of each tile (config 14): 8192x8192x127
         // (alpha 1.5, beta 0.5) 76 -> 98, 8192x8192x256 109 -> 116 TFLOP/s
-        // over config 2.
+        // over config 2.  With aligned columns and a full round of tiles the
+        // direct-loaded 128x128 tile (config 9) overtakes it above k = 160:
+        // 8192x8192x192 111 -> 114, x256 116 -> 123, 16384x16384x192 111 ->
+        // 122 (at k = 160 they tie, at 128 config 14 leads 101 to 96).
         const long long t128 = (long long)div_up(m, 128) * div_up(n, 128);
-        if (k <= 256 && t128 >= 512) return 14;
+        const int short_k = aligned && t128 >= 1024 ? 160 : 256;
+        if (k <= short_k && t128 >= 512) return 14;
         if (aligned) {
             if (t128 >= 1024) return 9;
             // 700-1023 tiles, short of a round of 1024: stream-K over all
