@@ -98,12 +98,33 @@ int heuristic(int form, int grouped, int m, int n, int k, bool aligned) {
         if (k <= short_k && t128 >= 512) return 14;
         if (aligned) {
             if (t128 >= 1024) return 9;
-            // 700-1023 tiles, short of a round of 1024: stream-K over all
-            // tiles (config 11, register-staged, 768 resident) for k >= 1024
-            // (3584^3 110 -> 131, 5120x2560x4096 113 -> 134 TFLOP/s), else
-            // 64x64 tiles (3584x3584x512 104 -> 114, 4096x3584x768 121 ->
-            // 125).
-            if (t128 >= 700) return k >= 1024 ? 11 : 7;
+            // Short of one round of 1024 direct-loaded 128x128 tiles, with k
+            // >= 1024 and from 180 tiles: every tile of a config takes the
+            // same time, so with c tiles per CU on average the busiest CU
+            // runs ceil(c) and a config reaches its full-round rate times
+            // c / ceil(c).  Full-round rates measured on 2048..4096 step 128:
+            // 128x128 (config 5) 141 TFLOP/s (0.83 of it with one tile per
+            // CU), 64x64 (config 7) 133, config 5's tile with split-K
+            // (config 10, splits to fill four workgroups per CU) 131 per
+            // split unit.  The largest modelled rate wins: within 2.5 % of
+            // the best config at every size of that sweep, where a fixed
+            // rule lost 7-10 % at 2432, 2944, 3456 and 3968 (e.g. 3968^3
+            // 123 -> 132, 2944^3 114 -> 123 TFLOP/s).
+            if (k >= 1024 && t128 >= 180) {
+                const double cus = device_cu_count();
+                auto fill = [](double c) { return c / std::ceil(c); };
+                const double c5 = (double)t128 / cus;
+                const double e5 = 141.0 * fill(c5) * (c5 <= 1.0 ? 0.83 : 1.0);
+                const double c7 = (double)div_up(m, 64) * div_up(n, 64) / cus;
+                const double e7 = 133.0 * fill(c7);
+                const long long s10 = std::llround(4.0 * cus / (double)t128);
+                const double e10 = s10 >= 2 ? 131.0 * fill((double)(t128 * s10) / cus) : 0.0;
+                if (e5 >= e7 && e5 >= e10) return 5;
+                return e10 > e7 ? 10 : 7;
+            }
+            // 700-1023 tiles with k < 1024: 64x64 tiles (3584x3584x512 104
+            // -> 114, 4096x3584x768 121 -> 125).
+            if (t128 >= 700) return 7;
         } else if (t128 >= 700) {
             return 9;
         }

@@ -90,18 +90,20 @@ def test_environment_table_loaded_on_first_use():
 
 def test_launch_config_follows_operand_alignment():
     """sgemm_launch_config reports what the default entry point launches
-    (pointers are not read).  FP32 with 16-byte aligned columns takes the
-    direct-to-LDS tile, 1024 resident, so 700-1023 tiles of 128x128 with
-    k >= 1024 go to stream-K (config 11); with odd leading dimensions the
-    register-staged tile (768 resident) takes the tail split (config 9)
-    from 700 tiles.  k == 0 and leading dimensions past 2^21 run no tile
-    config (-1)."""
+    (pointers are not read).  FP32 with 16-byte aligned columns short of a
+    round of 1024 direct-loaded 128x128 tiles picks by the fill model: at
+    3584^3 (784 tiles, 3.06 per CU, 12.25 64x64 tiles per CU) the 64x64
+    tile (config 7); at 3968^3 (961 tiles, 3.75 per CU) config 5.  With odd
+    leading dimensions the register-staged tile (768 resident) takes the
+    tail split (config 9) from 700 tiles.  k == 0 and leading dimensions
+    past 2^21 run no tile config (-1)."""
     F = sa.FORM_F32
     base = 1 << 20                                # 16-byte aligned fake addresses
     m = n = 3584                                  # 784 tiles of 128x128
     k = 3584
     aligned = sa.launch_config(F, m, n, k, base, m, base, k)
-    assert aligned == sa.autotune_lookup(F, m, n, k)[0] == 11
+    assert aligned == sa.autotune_lookup(F, m, n, k)[0] == 7
+    assert sa.launch_config(F, 3968, 3968, 3968, base, 3968, base, 3968) == 5
     assert sa.launch_config(F, m, n, k, base, m + 1, base, k) == 9      # odd lda
     assert sa.launch_config(F, m, n, k, base + 4, m, base, k) == 9      # A 4 bytes off
     assert sa.launch_config(F, 8192, 8192, 8192, base, 8192, base, 8192) == 9
