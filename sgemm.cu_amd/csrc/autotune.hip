@@ -173,9 +173,9 @@ int heuristic(int form, int grouped, int m, int n, int k, bool aligned) {
     struct Cand { int id, bm, bn, bk, per_cu; double rate; int split; };
     static const Cand cands[] = {
         {2, 256, 256, 32, 1, 1.00, kWhole},   {9, 256, 256, 32, 1, 1.00, kTailSplit},
-        {5, 256, 128, 16, 2, 0.92, kWhole},   {8, 128, 128, 32, 2, 0.84, kWhole},
-        {3, 256, 128, 32, 1, 0.69, kWhole},   {7, 64, 64, 64, 2, 0.47, kSplitK},
-        {10, 128, 128, 32, 2, 0.70, kSplitK},  // config 8's rate less its split's partials
+        {5, 256, 128, 16, 2, 0.82, kWhole},   {8, 128, 128, 32, 2, 0.73, kWhole},
+        {3, 256, 128, 32, 1, 0.60, kWhole},   {7, 64, 64, 64, 2, 0.41, kSplitK},
+        {10, 128, 128, 32, 2, 0.62, kSplitK},  // config 8's rate less its split's partials
         {11, 256, 256, 32, 1, 1.00, kStreamK},
     };
     const long long cus = device_cu_count();

@@ -51,9 +51,9 @@ int splitk_count(long long tiles, int k, int bk, bool f32 = false, int min_ktile
 int tail_split_count(long long tiles, long long slots, int per_cu, int k, int bk,
                      size_t split_bytes, double round_us, double* cost);
 
-// Useful BF16x3 rate of one CU running config 2's 256x256 tile (412 TFLOP/s
+// Useful BF16x3 rate of one CU running config 2's 256x256 tile (435 TFLOP/s
 // over 256 CUs at 8192^3), for the length of a round in the tail-split model.
-constexpr double kX3FlopsPerCuUs = 1.61e6;
+constexpr double kX3FlopsPerCuUs = 1.70e6;
 // Partials of one 256x256 BF16x3 split: 128 FP32 accumulators x 512 threads.
 constexpr size_t kX3TileSplitBytes = 256 * 256 * 4;
 // Minimum k-tiles per split of BF16x3 config 10 (the 128x128 tile of config

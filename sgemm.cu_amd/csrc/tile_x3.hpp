@@ -48,12 +48,13 @@ typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 // that many waves per SIMD fit (two 4-wave workgroups per CU: their barriers
 // fall at different times, so one's MFMAs fill the other's bubbles).
 template <int BM_, int BN_, int BK_, int WM_, int WN_, int MF_ = 16, bool DUAL_ = true,
-          bool MID_ = false, int WPE_ = 0>
+          bool MID_ = false, int WPE_ = 0, int SCHED_ = 0>
 struct X3Tile {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
     static constexpr int MF = MF_;
     static constexpr bool DUAL = DUAL_;
     static constexpr bool MID = MID_;
+    static constexpr int SCHED = SCHED_;
     // Waves per SIMD the kernel's registers must allow: two independent
     // workgroups per CU for the MID configs or when WPE_ asks for it, else
     // whatever the tile needs.
@@ -179,11 +180,11 @@ struct X3TileRunner {
             const auto ar = make_rsrc(P.A + a_base, 4 * (a_ext - a_base));
             const auto br = make_rsrc(P.B + b_base, 4 * (b_ext - b_base));
 #pragma unroll
+            for (int r = 0; r < T::B_VEC; ++r) rb[r] = bload4(br, b_off[r]);
+#pragma unroll
             for (int r = 0; r < T::A_SUB; ++r)
 #pragma unroll
                 for (int kk = 0; kk < 4; ++kk) ra[r][kk] = bload4(ar, a_off[r] + 4 * kk * P.lda);
-#pragma unroll
-            for (int r = 0; r < T::B_VEC; ++r) rb[r] = bload4(br, b_off[r]);
         };
         // MASK: zero k >= K (only the last k-tile can reach past K).
         auto sstore_impl = [&](char* stage, int k0, auto mask) {
@@ -224,6 +225,27 @@ struct X3TileRunner {
             if (k0 + BK <= K) sstore_impl(stage, k0, std::false_type{});
             else sstore_impl(stage, k0, std::true_type{});
         };
+        // One piece of a full k-tile's staging (the same values and
+        // addresses as sstore_impl without the mask): pieces [0, 4 * A_SUB)
+        // are the four rows of each A sub-block, the rest B's float4s.
+        auto sstore_piece = [&](char* stage, int c) {
+            if (c < 4 * T::A_SUB) {
+                const int r = c / 4, i = c % 4;
+                const f32x4 col = {ra[r][0][i], ra[r][1][i], ra[r][2][i], ra[r][3][i]};
+                bf16x4_t hi, lo;
+                split4(col, hi, lo);
+                const int off = (4 * (a_m4 + r * T::A_MSTEP) + i) * T::RS + 8 * a_k4;
+                *reinterpret_cast<bf16x4_t*>(stage + off) = hi;
+                *reinterpret_cast<bf16x4_t*>(stage + T::A_BYTES + off) = lo;
+            } else {
+                const int r = c - 4 * T::A_SUB;
+                bf16x4_t hi, lo;
+                split4(rb[r], hi, lo);
+                const int off = (b_n + r * T::B_NSTEP) * T::RS + 2 * b_k;
+                *reinterpret_cast<bf16x4_t*>(stage + 2 * T::A_BYTES + off) = hi;
+                *reinterpret_cast<bf16x4_t*>(stage + 2 * T::A_BYTES + T::B_BYTES + off) = lo;
+            }
+        };
 
         using Acc = typename std::conditional<T::MF == 16, f32x4, f32x16>::type;
         constexpr int NCRS = T::DUAL ? 1 : 0;
@@ -243,17 +265,107 @@ struct X3TileRunner {
         const int kbeg = pc.kbeg;
         const int kend = pc.kend;
         const int ktiles = kend > kbeg ? div_up(kend - kbeg, BK) : 0;
+        // SCHED != 0: iterations 0 .. nfast2-1 run interleaved_iteration,
+        // which needs k-tile t + 1 full and t + 2 present.
+        const int kfull = K - kbeg >= BK ? (K - kbeg) / BK : 0;  // full k-tiles from kbeg
+        const int nfast2 = T::SCHED != 0 && PEEL ? max(0, min(kfull - 1, ktiles - 2)) : 0;
         if (ktiles > 0) {
             gload(kbeg);
             sstore(smem, kbeg);
+            if (nfast2 > 0) {
+                // B first, then A: the order interleaved_iteration issues
+                // them in, so the counts of loads in flight match on entry
+                // to its loop from here and from its own last region.
+                const int k1 = kbeg + BK;
+                const long long a_base = m0 + (long long)k1 * P.lda;
+                const long long b_base = k1 + (long long)n0 * P.ldb;
+                const auto ar = make_rsrc(P.A + a_base, 4 * (a_ext - a_base));
+                const auto br = make_rsrc(P.B + b_base, 4 * (b_ext - b_base));
+#pragma unroll
+                for (int r = 0; r < T::B_VEC; ++r) rb[r] = bload4(br, b_off[r]);
+#pragma unroll
+                for (int r = 0; r < T::A_SUB; ++r)
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) ra[r][kk] = bload4(ar, a_off[r] + 4 * kk * P.lda);
+            }
             __syncthreads();
         }
+        // A full k-tile t with its loads one k-tile ahead (SCHED != 0).  On
+        // entry the staging registers hold k-tile t + 1 (loaded during
+        // iteration t - 1); the wave walks its MFMA blocks row by row in
+        // regions fenced by sched_barrier.  Region i reads the A operands of
+        // row i + 1 and runs row i's MFMAs (each product over all columns
+        // before the next, so consecutive MFMAs are independent; per
+        // accumulator the order stays al*bh, ah*bl, ah*bh).  Regions 0 ..
+        // B_VEC-1 also split B float4 i of k-tile t + 1 into the other LDS
+        // stage and at once reload its registers with k-tile t + 2; the last
+        // two regions split the A sub-block (it takes four loads, so its
+        // registers free up only after its last row) and reload it.  Each
+        // load so has about a whole k-tile to land, and the split runs in
+        // the shadow of the matrix cores.
+        auto interleaved_iteration = [&](int kt) {
+            if constexpr (T::SCHED != 0) {
+                static_assert(T::MF == 16 && BK == T::KSTEP && T::A_SUB == 1 &&
+                                  T::MB >= T::B_VEC + 2,
+                              "interleaved body: one k-step, B pieces before two A regions");
+                const char* stage = smem + (kt & 1) * T::STAGE;
+                char* next_stage = smem + ((kt + 1) & 1) * T::STAGE;
+                const char* Ahi = stage;
+                const char* Alo = stage + T::A_BYTES;
+                const char* Bhi = stage + 2 * T::A_BYTES;
+                const char* Blo = Bhi + T::B_BYTES;
+                const int k2 = kbeg + (kt + 2) * BK;
+                const long long a_base = m0 + (long long)k2 * P.lda;
+                const long long b_base = k2 + (long long)n0 * P.ldb;
+                const auto ar = make_rsrc(P.A + a_base, 4 * (a_ext - a_base));
+                const auto br = make_rsrc(P.B + b_base, 4 * (b_ext - b_base));
+                bf16x8_t ah[T::MB], al[T::MB], bh[T::NB], bl[T::NB];
+#pragma unroll
+                for (int j = 0; j < T::NB; ++j) {
+                    const int off = (wn * T::TN + 16 * j + lrow) * T::RS + 2 * lk;
+                    bh[j] = *reinterpret_cast<const bf16x8_t*>(Bhi + off);
+                    bl[j] = *reinterpret_cast<const bf16x8_t*>(Blo + off);
+                }
+                auto read_a = [&](int i) {
+                    const int off = (wm * T::TM + 16 * i + lrow) * T::RS + 2 * lk;
+                    ah[i] = *reinterpret_cast<const bf16x8_t*>(Ahi + off);
+                    al[i] = *reinterpret_cast<const bf16x8_t*>(Alo + off);
+                };
+                read_a(0);
+#pragma unroll
+                for (int i = 0; i < T::MB; ++i) {
+                    if (i + 1 < T::MB) read_a(i + 1);
+#pragma unroll
+                    for (int j = 0; j < T::NB; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+                    for (int j = 0; j < T::NB; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+                    for (int j = 0; j < T::NB; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                    if (i < T::B_VEC) {
+                        sstore_piece(next_stage, 4 + i);
+                        rb[i] = bload4(br, b_off[i]);
+                    }
+                    if (i >= T::MB - 2) {
+                        sstore_piece(next_stage, 2 * (i - (T::MB - 2)));
+                        sstore_piece(next_stage, 2 * (i - (T::MB - 2)) + 1);
+                    }
+                    if (i == T::MB - 1) {
+#pragma unroll
+                        for (int kk = 0; kk < 4; ++kk) ra[0][kk] = bload4(ar, a_off[0] + 4 * kk * P.lda);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+        };
         // One k-tile: MFMAs on the current stage while the next k-tile is
         // loaded, split and stored to the other stage.  NEXT_FULL: a next
         // k-tile exists and lies entirely below K, so the body needs no
         // masking and no branch -- one basic block, which the scheduling
         // hints below can interleave.
-        auto iteration = [&](int kt, auto next_full_tag) {
+        auto iteration = [&](int kt, auto next_full_tag, bool preloaded = false) {
             constexpr bool NEXT_FULL = decltype(next_full_tag)::value;
             const char* stage = smem + (kt & 1) * T::STAGE;
             char* next_stage = smem + ((kt + 1) & 1) * T::STAGE;
@@ -263,7 +375,7 @@ struct X3TileRunner {
             const char* Blo = Bhi + T::B_BYTES;
             const bool more = NEXT_FULL || kt + 1 < ktiles;
             const int knext = kbeg + (kt + 1) * BK;
-            if (more) gload(knext);
+            if (more && !preloaded) gload(knext);
             if constexpr (NEXT_FULL) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int kk = 0; kk < BK; kk += T::KSTEP) {
@@ -313,9 +425,9 @@ struct X3TileRunner {
                         // branch would): letting the scheduler mix it into
                         // the MFMA section raises register pressure past the
                         // budget.
-                        __builtin_amdgcn_sched_barrier(0);
+                        if constexpr (T::SCHED == 0) __builtin_amdgcn_sched_barrier(0);
                         sstore_impl(next_stage, knext, std::false_type{});
-                        __builtin_amdgcn_sched_barrier(0);
+                        if constexpr (T::SCHED == 0) __builtin_amdgcn_sched_barrier(0);
                     } else if (more) {
                         sstore(next_stage, knext);
                     }
@@ -326,10 +438,20 @@ struct X3TileRunner {
         // last one or two run the general body.  (Two loops in sequence, not
         // one loop with two bodies, so the accumulators never merge from two
         // paths -- that doubles their registers.)
-        const int kfull = K - kbeg >= BK ? (K - kbeg) / BK : 0;  // full k-tiles from kbeg
-        const int nfast = min(ktiles, kfull) - 1;              // iterations with a full successor
+        const int nfast = min(ktiles, kfull) - 1;  // iterations with a full successor
         int kt = 0;
-        if constexpr (PEEL) {
+        if constexpr (T::SCHED != 0 && PEEL) {
+            for (; kt < nfast2; ++kt) {
+                interleaved_iteration(kt);
+                __syncthreads();
+            }
+            // k-tile kt + 1 is already in the staging registers.
+            if (nfast2 > 0 && kt < ktiles) {
+                iteration(kt, std::false_type{}, true);
+                __syncthreads();
+                ++kt;
+            }
+        } else if constexpr (PEEL) {
             for (; kt < nfast; ++kt) {
                 iteration(kt, std::true_type{});
                 __syncthreads();
@@ -428,7 +550,10 @@ using X3_128x128 = X3Tile<128, 128, 32, 2, 2>;
 using X3_64x64 = X3Tile<64, 64, 64, 2, 2>;
 // 16x16x32 MFMAs on the 256x256 tile: 389 -> 413 TFLOP/s at 8192^3 and
 // 385 -> 398 at 16384^3 over 32x32x16 with scheduling hints.
-using X3_256x256_w8 = X3Tile<256, 256, 32, 2, 4, 16, false>;
+// The next k-tile's split interleaved with the MFMAs and its loads issued a
+// k-tile ahead (SCHED 1): 8192^3 404 -> 435, 16384^3 377 -> 423, 3072^3
+// 251 -> 292 TFLOP/s over the split staged after the MFMAs, same bits.
+using X3_256x256_w8 = X3Tile<256, 256, 32, 2, 4, 16, false, false, 0, 1>;
 using X3_256x128_w4 = X3Tile<256, 128, 32, 2, 2, 32, false>;
 using X3_128x128_m32 = X3Tile<128, 128, 32, 2, 2, 32, true>;
 using X3_256x128_k16 = X3Tile<256, 128, 16, 2, 2, 32, false, true>;
