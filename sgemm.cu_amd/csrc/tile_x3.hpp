@@ -46,7 +46,9 @@ typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 // when two independent workgroups share a CU and hurts a lone wave per SIMD,
 // whose loads then have less time to land.  WPE_ > 0 caps the registers so
 // that many waves per SIMD fit (two 4-wave workgroups per CU: their barriers
-// fall at different times, so one's MFMAs fill the other's bubbles).
+// fall at different times, so one's MFMAs fill the other's bubbles).  SCHED_
+// 1 runs full k-tiles through interleaved_iteration (X3TileRunner): the next
+// k-tile's split spread over the MFMAs, its loads a k-tile ahead.
 template <int BM_, int BN_, int BK_, int WM_, int WN_, int MF_ = 16, bool DUAL_ = true,
           bool MID_ = false, int WPE_ = 0, int SCHED_ = 0>
 struct X3Tile {
@@ -298,16 +300,16 @@ struct X3TileRunner {
         // before the next, so consecutive MFMAs are independent; per
         // accumulator the order stays al*bh, ah*bl, ah*bh).  Regions 0 ..
         // B_VEC-1 also split B float4 i of k-tile t + 1 into the other LDS
-        // stage and at once reload its registers with k-tile t + 2; the last
+        // stage and at once reload its registers with k-tile t + 2; the next
         // two regions split the A sub-block (it takes four loads, so its
-        // registers free up only after its last row) and reload it.  Each
-        // load so has about a whole k-tile to land, and the split runs in
-        // the shadow of the matrix cores.
+        // registers free up only after its last row), and the regions after
+        // them reload it.  Each load so has most of a k-tile to land, and
+        // the split runs in the shadow of the matrix cores.
         auto interleaved_iteration = [&](int kt) {
             if constexpr (T::SCHED != 0) {
                 static_assert(T::MF == 16 && BK == T::KSTEP && T::A_SUB == 1 &&
-                                  T::MB >= T::B_VEC + 2,
-                              "interleaved body: one k-step, B pieces before two A regions");
+                                  T::MB >= T::B_VEC + 3,
+                              "interleaved body: one k-step, B pieces, two A regions, A reloads");
                 const char* stage = smem + (kt & 1) * T::STAGE;
                 char* next_stage = smem + ((kt + 1) & 1) * T::STAGE;
                 const char* Ahi = stage;
@@ -348,14 +350,17 @@ struct X3TileRunner {
                         sstore_piece(next_stage, 4 + i);
                         rb[i] = bload4(br, b_off[i]);
                     }
-                    if (i >= T::MB - 2) {
-                        sstore_piece(next_stage, 2 * (i - (T::MB - 2)));
-                        sstore_piece(next_stage, 2 * (i - (T::MB - 2)) + 1);
+                    if (i >= T::B_VEC && i < T::B_VEC + 2) {
+                        sstore_piece(next_stage, 2 * (i - T::B_VEC));
+                        sstore_piece(next_stage, 2 * (i - T::B_VEC) + 1);
                     }
-                    if (i == T::MB - 1) {
+                    // One A reload per region from the one after the last A
+                    // row on (the rest in the last region), so no region
+                    // queues many loads at once.
 #pragma unroll
-                        for (int kk = 0; kk < 4; ++kk) ra[0][kk] = bload4(ar, a_off[0] + 4 * kk * P.lda);
-                    }
+                    for (int kk = 0; kk < 4; ++kk)
+                        if (i == min(T::B_VEC + 2 + kk, T::MB - 1))
+                            ra[0][kk] = bload4(ar, a_off[0] + 4 * kk * P.lda);
                     __builtin_amdgcn_sched_barrier(0);
                 }
             }
@@ -551,7 +556,7 @@ using X3_64x64 = X3Tile<64, 64, 64, 2, 2>;
 // 16x16x32 MFMAs on the 256x256 tile: 389 -> 413 TFLOP/s at 8192^3 and
 // 385 -> 398 at 16384^3 over 32x32x16 with scheduling hints.
 // The next k-tile's split interleaved with the MFMAs and its loads issued a
-// k-tile ahead (SCHED 1): 8192^3 404 -> 435, 16384^3 377 -> 423, 3072^3
+// k-tile ahead (SCHED 1): 8192^3 404 -> 439, 16384^3 377 -> 427, 3072^3
 // 251 -> 292 TFLOP/s over the split staged after the MFMAs, same bits.
 using X3_256x256_w8 = X3Tile<256, 256, 32, 2, 4, 16, false, false, 0, 1>;
 using X3_256x128_w4 = X3Tile<256, 128, 32, 2, 2, 32, false>;
