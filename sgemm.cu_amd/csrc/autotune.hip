@@ -158,8 +158,10 @@ int heuristic(int form, int grouped, int m, int n, int k, bool aligned) {
     // round costs per_cu x (tile area / splits) / rate, rate being the
     // config's relative speed measured at 8192^3.  In a last, partial round
     // a CU holds w < per_cu workgroups, which then run faster each: a lone
-    // one of a two-per-CU config gets about 60 % of the CU (measured at
-    // 2048^3 with config 8).  The cheapest config wins.  Configs 7 and 10
+    // one of a two-per-CU config gets `lone` of the CU -- about 60 % for
+    // configs 5 and 7, 78 % for 8 and 10, whose k loop overlaps the split
+    // with the MFMAs inside each wave (2048^3 on config 8: 253 TFLOP/s with
+    // one workgroup per CU against 326 at 8192^3).  The cheapest config wins.  Configs 7 and 10
     // split k when their grid is short of two workgroups per CU
     // (splitk_count); 10 only into splits of at least 16 k-tiles.
     // Config 9 is config 2 with a tail split: from one full round on, the
@@ -170,12 +172,12 @@ int heuristic(int form, int grouped, int m, int n, int k, bool aligned) {
     // round), costing what streamk_iters models: 3072^3 244 -> 314, 5120^3
     // 352 -> 387 TFLOP/s.
     enum { kWhole = 0, kSplitK = 1, kTailSplit = 2, kStreamK = 3 };
-    struct Cand { int id, bm, bn, bk, per_cu; double rate; int split; };
+    struct Cand { int id, bm, bn, bk, per_cu; double rate; int split; double lone; };
     static const Cand cands[] = {
-        {2, 256, 256, 32, 1, 1.00, kWhole},   {9, 256, 256, 32, 1, 1.00, kTailSplit},
-        {5, 256, 128, 16, 2, 0.82, kWhole},   {8, 128, 128, 32, 2, 0.73, kWhole},
-        {3, 256, 128, 32, 1, 0.60, kWhole},   {7, 64, 64, 64, 2, 0.41, kSplitK},
-        {10, 128, 128, 32, 2, 0.62, kSplitK},  // config 8's rate less its split's partials
+        {2, 256, 256, 32, 1, 1.00, kWhole, 1.0},   {9, 256, 256, 32, 1, 1.00, kTailSplit, 1.0},
+        {5, 256, 128, 16, 2, 0.81, kWhole, 0.6},   {8, 128, 128, 32, 2, 0.78, kWhole, 0.78},
+        {3, 256, 128, 32, 1, 0.62, kWhole, 1.0},   {7, 64, 64, 64, 2, 0.43, kSplitK, 0.6},
+        {10, 128, 128, 32, 2, 0.66, kSplitK, 0.78},  // config 8's rate less its split's partials
         {11, 256, 256, 32, 1, 1.00, kStreamK},
     };
     const long long cus = device_cu_count();
@@ -211,12 +213,19 @@ int heuristic(int form, int grouped, int m, int n, int k, bool aligned) {
                     ? splitk_count(tiles, k, c.bk, false, c.id == 10 ? kX3WideSplitKtiles : 4)
                     : 1;
             const long long wgs = tiles * splits;
-            const double work = (double)c.bm * c.bn / splits / c.rate;  // one workgroup, full CU share
+            // Config 10's partials cost about four k-tiles per split (written,
+            // then read back by the last split): 2048^3 (32 k-tiles per
+            // split) runs faster unsplit on config 8 (255 vs 245 TFLOP/s),
+            // 4096x1024x4096 (64 per split) faster split (299 vs 282).
+            const double split_cost =
+                c.id == 10 && splits > 1 ? 1.0 + 4.0 / ((double)k / splits / c.bk) : 1.0;
+            const double work = (double)c.bm * c.bn / splits / c.rate * split_cost;  // one workgroup, full CU share
             cost = (double)(wgs / slots) * c.per_cu * work;
             const long long rem = wgs % slots;
             if (rem > 0) {
                 const long long w = div_up(rem, cus);
-                const double u = w >= c.per_cu ? 1.0 : 0.6 + 0.4 * (double)(w - 1) / (c.per_cu - 1);
+                const double u =
+                    w >= c.per_cu ? 1.0 : c.lone + (1.0 - c.lone) * (double)(w - 1) / (c.per_cu - 1);
                 cost += (double)w * work / u;
             }
         }

@@ -307,8 +307,10 @@ struct X3TileRunner {
         // the split runs in the shadow of the matrix cores.
         auto interleaved_iteration = [&](int kt) {
             if constexpr (T::SCHED != 0) {
+                constexpr int SUB = T::MB >= 8 ? 1 : 8 / T::MB;
+                constexpr int NREG = T::MB * SUB;
                 static_assert(T::MF == 16 && BK == T::KSTEP && T::A_SUB == 1 &&
-                                  T::MB >= T::B_VEC + 3,
+                                  T::NB % SUB == 0 && NREG >= T::B_VEC + 3,
                               "interleaved body: one k-step, B pieces, two A regions, A reloads");
                 const char* stage = smem + (kt & 1) * T::STAGE;
                 char* next_stage = smem + ((kt + 1) & 1) * T::STAGE;
@@ -334,32 +336,36 @@ struct X3TileRunner {
                     al[i] = *reinterpret_cast<const bf16x8_t*>(Alo + off);
                 };
                 read_a(0);
+                // NREG regions: row i of blocks in SUB parts of NB / SUB
+                // columns each (SUB = 2 for a four-row wave tile, so there are
+                // eight regions either way).
 #pragma unroll
-                for (int i = 0; i < T::MB; ++i) {
-                    if (i + 1 < T::MB) read_a(i + 1);
+                for (int r = 0; r < NREG; ++r) {
+                    const int i = r / SUB, j0 = (r % SUB) * (T::NB / SUB);
+                    if (r % SUB == 0 && i + 1 < T::MB) read_a(i + 1);
 #pragma unroll
-                    for (int j = 0; j < T::NB; ++j)
+                    for (int j = j0; j < j0 + T::NB / SUB; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
 #pragma unroll
-                    for (int j = 0; j < T::NB; ++j)
+                    for (int j = j0; j < j0 + T::NB / SUB; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
 #pragma unroll
-                    for (int j = 0; j < T::NB; ++j)
+                    for (int j = j0; j < j0 + T::NB / SUB; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-                    if (i < T::B_VEC) {
-                        sstore_piece(next_stage, 4 + i);
-                        rb[i] = bload4(br, b_off[i]);
+                    if (r < T::B_VEC) {
+                        sstore_piece(next_stage, 4 + r);
+                        rb[r] = bload4(br, b_off[r]);
                     }
-                    if (i >= T::B_VEC && i < T::B_VEC + 2) {
-                        sstore_piece(next_stage, 2 * (i - T::B_VEC));
-                        sstore_piece(next_stage, 2 * (i - T::B_VEC) + 1);
+                    if (r >= T::B_VEC && r < T::B_VEC + 2) {
+                        sstore_piece(next_stage, 2 * (r - T::B_VEC));
+                        sstore_piece(next_stage, 2 * (r - T::B_VEC) + 1);
                     }
                     // One A reload per region from the one after the last A
                     // row on (the rest in the last region), so no region
                     // queues many loads at once.
 #pragma unroll
                     for (int kk = 0; kk < 4; ++kk)
-                        if (i == min(T::B_VEC + 2 + kk, T::MB - 1))
+                        if (r == min(T::B_VEC + 2 + kk, NREG - 1))
                             ra[0][kk] = bload4(ar, a_off[0] + 4 * kk * P.lda);
                     __builtin_amdgcn_sched_barrier(0);
                 }
@@ -564,7 +570,9 @@ using X3_128x128_m32 = X3Tile<128, 128, 32, 2, 2, 32, true>;
 using X3_256x128_k16 = X3Tile<256, 128, 16, 2, 2, 32, false, true>;
 // 16x16x32 MFMAs: the grouped BF16x3 benchmark 97 -> 102 TFLOP/s, config
 // 10 at 2048^3 246 -> 263 TFLOP/s over 32x32x16.
-using X3_128x128_2cu = X3Tile<128, 128, 32, 2, 2, 16, false, false, 2>;
+// Interleaved k loop as config 2's (SCHED 1): 2048^3 200 -> 275, 1536^3 112
+// -> 153, 8192^3 316 -> 338 TFLOP/s, same bits.
+using X3_128x128_2cu = X3Tile<128, 128, 32, 2, 2, 16, false, false, 2, 1>;
 
 template <class F>
 __host__ __device__ inline bool dispatch_x3(int id, F&& f) {
