@@ -176,6 +176,7 @@ struct X3TileRunner {
         // Raw loads land in ra/rb; k >= K masking and the split happen in
         // sstore, after the MFMAs of the current tile.
         f32x4 ra[T::A_SUB][4], rb[T::B_VEC];
+        // B's loads first, then A's (see the prologue).
         auto gload = [&](int k0) {
             const long long a_base = m0 + (long long)k0 * P.lda;
             const long long b_base = k0 + (long long)n0 * P.ldb;
@@ -274,22 +275,11 @@ struct X3TileRunner {
         if (ktiles > 0) {
             gload(kbeg);
             sstore(smem, kbeg);
-            if (nfast2 > 0) {
-                // B first, then A: the order interleaved_iteration issues
-                // them in, so the counts of loads in flight match on entry
-                // to its loop from here and from its own last region.
-                const int k1 = kbeg + BK;
-                const long long a_base = m0 + (long long)k1 * P.lda;
-                const long long b_base = k1 + (long long)n0 * P.ldb;
-                const auto ar = make_rsrc(P.A + a_base, 4 * (a_ext - a_base));
-                const auto br = make_rsrc(P.B + b_base, 4 * (b_ext - b_base));
-#pragma unroll
-                for (int r = 0; r < T::B_VEC; ++r) rb[r] = bload4(br, b_off[r]);
-#pragma unroll
-                for (int r = 0; r < T::A_SUB; ++r)
-#pragma unroll
-                    for (int kk = 0; kk < 4; ++kk) ra[r][kk] = bload4(ar, a_off[r] + 4 * kk * P.lda);
-            }
+            // k-tile 1 for the first interleaved iteration.  gload issues B,
+            // then A, as interleaved_iteration does, so the counts of loads in
+            // flight match on entry to its loop from here and from its own
+            // last region (the waits at its head take the stricter of the two).
+            if (nfast2 > 0) gload(kbeg + BK);
             __syncthreads();
         }
         // A full k-tile t with its loads one k-tile ahead (SCHED != 0).  On

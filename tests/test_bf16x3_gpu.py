@@ -408,3 +408,20 @@ def test_streamk_concurrent_streams_match_sequential():
         torch.cuda.synchronize()
         for i, (_, _, dC) in enumerate(devs):
             assert np.array_equal(download(dC), alone[i]), (rep, i)
+
+
+# ------------------------------------------------- single-set tiles agree
+@pytest.mark.parametrize("m,n,k,pad", [(300, 260, 1000, 3), (513, 257, 97, 0), (256, 512, 4096, 1)])
+def test_single_set_tiles_agree_bit_for_bit(m, n, k, pad):
+    """Configs 2 (256x256, eight waves, each block row a region of its
+    interleaved k loop) and 8 (128x128, two per CU, each block row split into
+    two regions) accumulate every C element as al*bh, ah*bl, ah*bh per 32-deep
+    k-tile in k order, so their results are the same bits; the shapes run the
+    interleaved loop, its hand-over to the general loop and a ragged last
+    k-tile."""
+    c = Case(m, n, k, lda=m + pad, ldb=k + pad, ldc=m + pad, kind="wide", seed=m * 7 + k,
+             alpha=1.5, beta=0.5)
+    a = run_form(c, sa.FORM_BF16X3, 2)
+    b = run_form(c, sa.FORM_BF16X3, 8)
+    assert_ok(check(c, a, orc.bf16x3_tol_mult(k)), f"config 2 {(m, n, k)}")
+    assert np.array_equal(a, b), (m, n, k)
