@@ -115,7 +115,12 @@ int sgemm_ex(int m, int n, int k, const float* alpha, const float* A, int lda, c
  * (the tiles of a partial round spread k-tile by k-tile over the CUs, cut
  * tiles summed from their pieces in k order, so also deterministic), with the
  * same per-stream workspace and capture behaviour; its partials take up to
- * 512 MiB. */
+ * 512 MiB.  For large problems it splits A and B once, in a pass of their own,
+ * into packed hi/lo planes held in the stream's workspace (about the size of
+ * A and B, up to 16 GiB; config 12); where that workspace cannot be had (a
+ * stream under capture whose workspace would have to grow, an allocation
+ * failure, larger problems) the same tile runs from A and B directly (config
+ * 2), with the same result bits. */
 int sgemm_bf16x3(int m, int n, int k, const float* alpha, const float* A, int lda,
                  const float* B, int ldb, const float* beta, float* C, int ldc,
                  sgemm_stream_t stream);
@@ -125,7 +130,9 @@ int sgemm_bf16x3(int m, int n, int k, const float* alpha, const float* A, int ld
  * no device state); for callers that cannot allocate device memory: code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc.  Same arguments, config choice, kernels and result bits as those
- * entry points, but the split-K, tail-split and stream-K partials and arrival
+ * entry points (except that a packed BF16x3 pick, config 12, runs config 2's
+ * tile from A and B, which gives the same bits), but the split-K, tail-split
+ * and stream-K partials and arrival
  * counters come from one static device array per GPU, held until the process
  * ends (SGEMM_NOALLOC_SCRATCH_PARTIAL_BYTES of partials and
  * SGEMM_NOALLOC_SCRATCH_COUNTERS counters), instead of a per-stream
@@ -141,11 +148,11 @@ int sgemm_ex_noalloc(int form, int m, int n, int k, const float* alpha, const fl
                      const float* B, int ldb, const float* beta, float* C, int ldc,
                      sgemm_stream_t stream);
 
-/* Frees the split-K / stream-K workspace the library holds for `stream`
- * (waiting for the stream first).  Call it before destroying a stream that
- * ran split configs, so the handle cannot be reused while its workspace is
- * still in use; graphs that captured split launches on `stream` must not be
- * replayed afterwards.  A stream without a workspace is a no-op.  No
+/* Frees the split-K / stream-K workspace and the packed BF16x3 planes the
+ * library holds for `stream` (waiting for the stream first).  Call it before
+ * destroying a stream that ran split or packed configs, so the handle cannot
+ * be reused while its workspace is still in use; graphs that captured such
+ * launches on `stream` must not be replayed afterwards.  A stream without a workspace is a no-op.  No
  * reference counterpart (the reference holds no device state). */
 int sgemm_workspace_release(sgemm_stream_t stream);
 
@@ -270,7 +277,8 @@ typedef struct {
                                 interleaved with the small-request batches (the config the
                                 autotune table picks for the panel's shape, except that a
                                 tail-split or stream-K pick runs the same tile whole, since
-                                other slots share the GPU); 0 = 100 GFLOP, < 0 = never cut
+                                other slots share the GPU, and a packed BF16x3 pick runs
+                                config 2, whose bits it shares); 0 = 100 GFLOP, < 0 = never cut
                                 (every request goes into grouped batches) */
     int mock_fail_launch;    /* SGEMM_EXECUTOR_MOCK only: the launch with this 1-based index
                                 (grouped batch or panel) fails, to test failure handling;

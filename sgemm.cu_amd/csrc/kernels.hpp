@@ -67,10 +67,13 @@ constexpr int kX3WideSplitKtiles = 16;
 constexpr double kX3StreamKPieceIters = 7.0;
 
 // Configs that balance a partial last round of tiles over the whole GPU --
-// the tail split (9) and stream-K (11) of either form -- and the config that
-// runs the same tile whole (FP32 5, BF16x3 2); -1 for any other config.
+// the tail split (9) and stream-K (11) of either form -- and BF16x3's packed
+// config (12, which holds a pack workspace of the operands' size), mapped to
+// the config that runs the same tile whole from the operands themselves
+// (FP32 5, BF16x3 2, with the same result bits); -1 for any other config.
 constexpr int whole_tile_twin(int form, int config) {
-    return (config == 9 || config == 11) ? (form == kFormBf16x3 ? 2 : 5) : -1;
+    if (form == kFormBf16x3) return (config == 9 || config == 11 || config == 12) ? 2 : -1;
+    return (config == 9 || config == 11) ? 5 : -1;
 }
 
 // The same for plain FP32 config 11 (config 5's 128x128x16 tile, three
@@ -119,6 +122,13 @@ hipError_t splitk_workspace(hipStream_t stream, size_t partial_bytes, int tiles,
 // Frees the stream's workspace (after waiting for the stream).
 hipError_t splitk_workspace_release(hipStream_t stream);
 
+// The stream's buffer for BF16x3 config 12's packed operand planes, at least
+// `bytes` long: grown on demand like the split workspace (outgrown buffers
+// are kept for captured graphs until release), never under capture
+// (hipErrorStreamCaptureUnsupported), hipErrorOutOfMemory when the
+// allocation fails.  Its contents do not outlive a launch.
+hipError_t pack_workspace(hipStream_t stream, size_t bytes, void** out);
+
 // Configs the persistent grouped kernel carries (grouped.hip).  Its
 // registers and LDS are the maximum over these, which sets its occupancy:
 // FP32 {1, 2, 5} fit three workgroups per CU, BF16x3 {1, 8} two.
@@ -137,6 +147,15 @@ int bf16x3_config_count();
 const ConfigInfo& bf16x3_config(int id);
 hipError_t launch_bf16x3(int id, const Problem& p, hipStream_t stream,
                          const Arena* arena = nullptr);
+// BF16x3 config 12 (packed_x3.hip): split once into packed planes, then the
+// packed 256x256 tile; runs config 2 (same bits) where the planes cannot be
+// had (capture, arena, size).
+hipError_t launch_x3_packed(const Problem& p, hipStream_t stream, const Arena* arena);
+// Packed-plane rate of config 12's k loop relative to config 2's (GEMM part,
+// 8192^3: 527 against 439 TFLOP/s) and the pack's read + write rate in bytes
+// per microsecond, for the cost model.
+constexpr double kX3PackedRate = 1.20;
+constexpr double kX3PackBytesPerUs = 4.8e6;
 
 // Naive one-thread-per-output kernel (sgemm_f32.hip).
 hipError_t launch_basic(const Problem& p, hipStream_t stream);

@@ -638,7 +638,8 @@ class Server {
         // every CU, which assumes the launch has the GPU to itself; here
         // other slots' work shares it.  Such picks run the same tile whole:
         // at the benchmark's mixture that raised the capacity from 6450 to
-        // 8400 requests/s.
+        // 8400 requests/s.  A packed BF16x3 pick runs config 2 too (same
+        // bits): its planes would be device memory outside the budget.
         int cfg = -1, from_table = 0;
         int whole = -1;
         if (sgemm_autotune_lookup(cfg_.form, 0, r.m, nc, r.k, &cfg, &from_table) ==

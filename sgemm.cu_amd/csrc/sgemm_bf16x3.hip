@@ -61,6 +61,8 @@ void k_sgemm_bf16x3_sk(Problem P, SplitK sk) {
 // runs the tile of id 8 with split-K, each split at least 16 k-tiles deep
 // (shorter splits of this tile lose to the 64x64 one).  Id 11 runs the tile
 // of id 2 with stream-K over the tiles of a partial last round (splitk = 3).
+// Id 12 splits the operands once into packed planes and runs a 256x256 tile
+// on them (packed_x3.hip), with id 2's result bits.
 static const ConfigInfo kBf16x3Configs[] = {
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m16_dual"},
     {64, 64, 64, 2, 2, "bf16x3_64x64x64_w2x2_m16_dual"},
@@ -74,6 +76,7 @@ static const ConfigInfo kBf16x3Configs[] = {
     {256, 256, 32, 2, 4, "bf16x3_256x256x32_w2x4_m16_single_tailsplit", 2},
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m16_single_2cu_splitk", 1},
     {256, 256, 32, 2, 4, "bf16x3_256x256x32_w2x4_m16_single_streamk", 3},
+    {256, 256, 32, 2, 4, "bf16x3_256x256x32_w2x4_m16_single_packed"},
 };
 
 int bf16x3_config_count() { return (int)(sizeof(kBf16x3Configs) / sizeof(kBf16x3Configs[0])); }
@@ -215,6 +218,7 @@ hipError_t launch_bf16x3(int id, const Problem& p, hipStream_t s, const Arena* a
         case 9: return launch_x3_tail<X3_256x256_w8>(p, s, arena);
         case 10: return launch_x3<X3_128x128_2cu, true, kX3WideSplitKtiles>(p, s, arena);
         case 11: return launch_x3_streamk<X3_256x256_w8>(p, s, arena);
+        case 12: return launch_x3_packed(p, s, arena);
         default: break;
     }
     hipError_t e = hipErrorInvalidValue;

@@ -1,4 +1,5 @@
-// workspace.cpp -- split-K workspace per stream and the split count rule.
+// workspace.cpp -- split-K workspace per stream (with the BF16x3 packed
+// operand planes) and the split count rule.
 //
 // Split-K partials and the per-tile arrival counters live in device memory
 // owned by the library: one workspace per stream (launches on one stream are
@@ -39,8 +40,17 @@ struct Workspace {
     size_t partial_bytes = 0;
     int* counters = nullptr;
     int counter_count = 0;
+    void* packed = nullptr;      // BF16x3 config 12's packed operand planes
+    size_t packed_bytes = 0;
     std::vector<void*> retired;  // outgrown buffers, kept for captured graphs
 };
+
+// True when `stream` is being captured into a graph.
+bool capturing(hipStream_t stream) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cs) != hipSuccess) (void)hipGetLastError();
+    return cs != hipStreamCaptureStatusNone;
+}
 
 std::mutex g_mu;
 std::map<hipStream_t, Workspace> g_ws;
@@ -187,11 +197,8 @@ hipError_t splitk_workspace(hipStream_t stream, size_t partial_bytes, int tiles,
     }
     std::lock_guard<std::mutex> lock(g_mu);
     Workspace& w = g_ws[stream];
-    if (w.partial_bytes < partial_bytes || w.counter_count < tiles) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        if (hipStreamIsCapturing(stream, &cs) != hipSuccess) (void)hipGetLastError();
-        if (cs != hipStreamCaptureStatusNone) return hipErrorStreamCaptureUnsupported;
-    }
+    if ((w.partial_bytes < partial_bytes || w.counter_count < tiles) && capturing(stream))
+        return hipErrorStreamCaptureUnsupported;
     if (w.partial_bytes < partial_bytes) {
         const size_t bytes = std::max(partial_bytes + partial_bytes / 4, 2 * w.partial_bytes);
         void* p = nullptr;
@@ -228,6 +235,25 @@ hipError_t splitk_workspace(hipStream_t stream, size_t partial_bytes, int tiles,
     return hipSuccess;
 }
 
+hipError_t pack_workspace(hipStream_t stream, size_t bytes, void** out) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    Workspace& w = g_ws[stream];
+    if (w.packed_bytes < bytes) {
+        if (capturing(stream)) return hipErrorStreamCaptureUnsupported;
+        const size_t grow = std::max(bytes + bytes / 4, 2 * w.packed_bytes);
+        void* p = nullptr;
+        if (hipMalloc(&p, grow) != hipSuccess) {
+            (void)hipGetLastError();
+            return hipErrorOutOfMemory;
+        }
+        if (w.packed) w.retired.push_back(w.packed);
+        w.packed = p;
+        w.packed_bytes = grow;
+    }
+    *out = w.packed;
+    return hipSuccess;
+}
+
 hipError_t splitk_workspace_release(hipStream_t stream) {
     Workspace w;
     {
@@ -241,6 +267,7 @@ hipError_t splitk_workspace_release(hipStream_t stream) {
     hipError_t e = hipStreamSynchronize(stream);
     if (w.partials) (void)hipFree(w.partials);
     if (w.counters) (void)hipFree(w.counters);
+    if (w.packed) (void)hipFree(w.packed);
     for (void* p : w.retired) (void)hipFree(p);
     return e;
 }

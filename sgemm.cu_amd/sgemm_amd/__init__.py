@@ -282,7 +282,8 @@ def run_config(form, config_id, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st
 
 
 def workspace_release(stream=None) -> None:
-    """Free the library's split-K workspace of `stream` (sgemm_workspace_release)."""
+    """Free the library's split-K workspace and packed BF16x3 planes of `stream`
+    (sgemm_workspace_release)."""
     _check(lib().sgemm_workspace_release(_stream(stream)), "sgemm_workspace_release")
 
 

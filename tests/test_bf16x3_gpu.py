@@ -22,7 +22,8 @@ from gpu_util import (Case, assert_ok, check, compact, download, log, padding_un
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gemm_cases.json")
-N_X3_CONFIGS = 12
+N_X3_CONFIGS = 13
+X3_PACKED_CONFIG = 12
 
 
 def test_config_table():
@@ -425,3 +426,135 @@ def test_single_set_tiles_agree_bit_for_bit(m, n, k, pad):
     b = run_form(c, sa.FORM_BF16X3, 8)
     assert_ok(check(c, a, orc.bf16x3_tol_mult(k)), f"config 2 {(m, n, k)}")
     assert np.array_equal(a, b), (m, n, k)
+
+
+@pytest.mark.parametrize("m,n,k,pad", [(300, 260, 1000, 3), (513, 257, 97, 0), (1, 1, 1, 0),
+                                       (17, 300, 33, 2), (256, 512, 4096, 1), (640, 300, 2049, 0)])
+def test_packed_matches_config2_bit_for_bit(m, n, k, pad):
+    """Config 12 splits A and B once into packed hi/lo planes (rows past the
+    operand and k past K zero) and runs the 256x256 tile on them; every
+    accumulator takes al*bh, ah*bl, ah*bh per 32-deep k-tile in k order, as
+    config 2 does, so the bits agree.  Ragged panels and k-tiles, padded
+    leading dimensions, and the FP64 oracle."""
+    c = Case(m, n, k, lda=m + pad, ldb=k + pad, ldc=m + pad, kind="wide", seed=m * 5 + k,
+             alpha=1.5, beta=0.5)
+    a = run_form(c, sa.FORM_BF16X3, 2)
+    b = run_form(c, sa.FORM_BF16X3, X3_PACKED_CONFIG)
+    assert_ok(check(c, b, orc.bf16x3_tol_mult(k)), f"config 12 {(m, n, k)}")
+    assert np.array_equal(a, b), (m, n, k)
+
+
+def test_packed_large_ragged_matches_config2():
+    """Several panels and a partial last panel in both m and n, k not a
+    multiple of 32, odd leading dimensions: the same bits as config 2 (which
+    the oracle checks on this shape in the sweep tests)."""
+    c = Case(4097, 3071, 2049, kind="wide", seed=311, alpha=1.5, beta=0.5)
+    a = run_form(c, sa.FORM_BF16X3, 2)
+    b = run_form(c, sa.FORM_BF16X3, X3_PACKED_CONFIG)
+    assert np.array_equal(a, b)
+
+
+def test_packed_non_finite_and_beta_zero_nan_c():
+    """+-Inf, NaN and values beyond the BF16 range pass through the pack's
+    split as through config 2's (same bits, NaN included); with beta = 0 a
+    NaN-filled C is not read."""
+    c = Case(300, 280, 200, kind="wide", seed=5, alpha=1.0, beta=0.0, c_fill=np.nan)
+    c.A[[0, 7, 1000, 2001]] = [np.inf, -np.inf, np.nan, np.float32(3.4e38)]
+    c.B[[3, 500, 777]] = [np.float32(-3.4e38), -np.inf, np.float32(1e-40)]
+    a = run_form(c, sa.FORM_BF16X3, 2)
+    b = run_form(c, sa.FORM_BF16X3, X3_PACKED_CONFIG)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    clean = Case(300, 280, 200, kind="wide", seed=6, alpha=1.0, beta=0.0, c_fill=np.nan)
+    out = run_form(clean, sa.FORM_BF16X3, X3_PACKED_CONFIG)
+    assert not np.isnan(compact(out, 300, 280, 300)).any()
+    assert_ok(check(clean, out, orc.bf16x3_tol_mult(200)), "config 12 beta 0")
+
+
+def test_packed_graph_replays_and_capture_fallback():
+    """Captured after a warm-up call on the stream (its pack workspace then
+    exists), config 12 replays bit-identically to its eager result, with
+    eager work between replays; captured on a fresh stream it allocates
+    nothing and runs config 2, whose bits are the same."""
+    import torch
+    m, n, k = 1500, 1300, 1100
+    c = Case(m, n, k, seed=83, alpha=1.0, beta=0.0)
+    dA, dB, dC = c.device()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        sa.run_config(1, X3_PACKED_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, s)
+        s.synchronize()
+        eager = download(dC).copy()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            sa.run_config(1, X3_PACKED_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m,
+                          torch.cuda.current_stream())
+    other = Case(700, 600, 500, seed=84)
+    oA, oB, oC = other.device()
+    for rep in range(3):
+        dC.fill_(5.0)
+        sa.run_config(1, X3_PACKED_CONFIG, 700, 600, 500, 1.0, oA, 700, oB, 500, 0.0, oC, 700)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(download(dC), eager), rep
+    fresh = torch.cuda.Stream()
+    g2 = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g2, stream=fresh):
+        sa.run_config(1, X3_PACKED_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m,
+                      torch.cuda.current_stream())
+    for rep in range(2):
+        dC.fill_(-2.0)
+        torch.cuda.synchronize()
+        g2.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(download(dC), eager), rep
+    sa.workspace_release(s)
+
+
+def test_packed_concurrent_streams_match_sequential():
+    """Each stream has its own packed planes: two config-12 launches of
+    different problems running at once on two streams give the bits each
+    gives alone."""
+    import torch
+    shapes = [(2000, 2100, 1536), (1800, 1700, 2048)]
+    cases = [Case(m, n, k, seed=95 + i, alpha=1.0, beta=0.0) for i, (m, n, k) in enumerate(shapes)]
+    devs = [c.device() for c in cases]
+    alone = []
+    for c, (dA, dB, dC) in zip(cases, devs):
+        sa.run_config(1, X3_PACKED_CONFIG, c.m, c.n, c.k, 1.0, dA, c.m, dB, c.k, 0.0, dC, c.m)
+        torch.cuda.synchronize()
+        alone.append(download(dC).copy())
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for rep in range(3):
+        for (dA, dB, dC) in devs:
+            dC.fill_(1.0)
+        torch.cuda.synchronize()
+        for c, (dA, dB, dC), st in zip(cases, devs, streams):
+            sa.run_config(1, X3_PACKED_CONFIG, c.m, c.n, c.k, 1.0, dA, c.m, dB, c.k, 0.0, dC,
+                          c.m, st)
+        torch.cuda.synchronize()
+        for i, (dA, dB, dC) in enumerate(devs):
+            assert np.array_equal(download(dC), alone[i]), (rep, i)
+    for st in streams:
+        sa.workspace_release(st)
+
+
+def test_default_pick_packed_and_noalloc_same_bits():
+    """At 8192 x 8192 x 1024 the default BF16x3 entry point picks config 12;
+    its result equals config 2's and that of the no-allocation entry point
+    (which runs config 2's tile from A and B) bit for bit."""
+    import torch
+    m, n, k = 8192, 8192, 1024
+    c = Case(m, n, k, seed=17, alpha=1.0, beta=0.0, c_fill=0.0)
+    dA, dB, dC = c.device()
+    assert sa.launch_config(sa.FORM_BF16X3, m, n, k, dA, m, dB, k) == X3_PACKED_CONFIG
+    log("default bf16x3 8192x8192x1024")
+    sa.sgemm_bf16x3(m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m)
+    default = download(dC).copy()
+    sa.run_config(1, 2, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m)
+    assert np.array_equal(download(dC), default)
+    dC.fill_(3.0)
+    sa.sgemm_ex_noalloc(sa.FORM_BF16X3, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m)
+    assert np.array_equal(download(dC), default)
+    sa.workspace_release(None)
+    torch.cuda.synchronize()
