@@ -171,19 +171,20 @@ int heuristic(int form, int grouped, int m, int n, int k, bool aligned) {
     // with stream-K over those tiles (or over all tiles, short of a full
     // round), costing what streamk_iters models: 3072^3 244 -> 314, 5120^3
     // 352 -> 387 TFLOP/s.
-    // Config 12 is config 2's tile on operands split once into packed
-    // planes: its k loop runs kX3PackedRate times config 2's, plus the pack
-    // pass (A and B read, their padded planes written, at
+    // Configs 12 and 13 are configs 2 and 11 on operands split once into
+    // packed planes: their k loop runs kX3PackedRate times config 2's, plus
+    // the pack pass (A and B read, their padded planes written, at
     // kX3PackBytesPerUs, and two launches more).
-    enum { kWhole = 0, kSplitK = 1, kTailSplit = 2, kStreamK = 3, kPacked = 4 };
-    struct Cand { int id, bm, bn, bk, per_cu; double rate; int split; double lone; };
+    enum { kWhole = 0, kSplitK = 1, kTailSplit = 2, kStreamK = 3 };
+    struct Cand { int id, bm, bn, bk, per_cu; double rate; int split; double lone; bool packed; };
     static const Cand cands[] = {
         {2, 256, 256, 32, 1, 1.00, kWhole, 1.0},   {9, 256, 256, 32, 1, 1.00, kTailSplit, 1.0},
         {5, 256, 128, 16, 2, 0.81, kWhole, 0.6},   {8, 128, 128, 32, 2, 0.78, kWhole, 0.78},
         {3, 256, 128, 32, 1, 0.62, kWhole, 1.0},   {7, 64, 64, 64, 2, 0.43, kSplitK, 0.6},
         {10, 128, 128, 32, 2, 0.66, kSplitK, 0.78},  // config 8's rate less its split's partials
-        {11, 256, 256, 32, 1, 1.00, kStreamK},
-        {12, 256, 256, 32, 1, kX3PackedRate, kPacked, 1.0},
+        {11, 256, 256, 32, 1, 1.00, kStreamK, 1.0},
+        {12, 256, 256, 32, 1, kX3PackedRate, kWhole, 1.0, true},
+        {13, 256, 256, 32, 1, kX3PackedRate, kStreamK, 1.0, true},
     };
     const long long cus = device_cu_count();
     int best = 7;
@@ -233,14 +234,14 @@ int heuristic(int form, int grouped, int m, int n, int k, bool aligned) {
                     w >= c.per_cu ? 1.0 : c.lone + (1.0 - c.lone) * (double)(w - 1) / (c.per_cu - 1);
                 cost += (double)w * work / u;
             }
-            if (c.split == kPacked) {
-                // In the cost's unit (tile area at config 2's rate, 2 k /
-                // kX3FlopsPerCuUs microseconds).
-                const double kp = 32.0 * div_up(k, 32);
-                const double bytes = 4.0 * ((double)m + n) * k +
-                                     4.0 * (256.0 * div_up(m, 256) + 256.0 * div_up(n, 256)) * kp;
-                cost += (bytes / kX3PackBytesPerUs + 5.0) / (2.0 * k / kX3FlopsPerCuUs);
-            }
+        }
+        if (c.packed) {
+            // In the cost's unit (tile area at config 2's rate, 2 k /
+            // kX3FlopsPerCuUs microseconds).
+            const double kp = 32.0 * div_up(k, 32);
+            const double bytes = 4.0 * ((double)m + n) * k +
+                                 4.0 * (256.0 * div_up(m, 256) + 256.0 * div_up(n, 256)) * kp;
+            cost += (bytes / kX3PackBytesPerUs + 5.0) / (2.0 * k / kX3FlopsPerCuUs);
         }
         if (best_cost == 0 || cost < best_cost) {
             best = c.id;

@@ -68,11 +68,13 @@ constexpr double kX3StreamKPieceIters = 7.0;
 
 // Configs that balance a partial last round of tiles over the whole GPU --
 // the tail split (9) and stream-K (11) of either form -- and BF16x3's packed
-// config (12, which holds a pack workspace of the operands' size), mapped to
+// configs (12, 13, which hold a pack workspace of the operands' size), mapped to
 // the config that runs the same tile whole from the operands themselves
-// (FP32 5, BF16x3 2, with the same result bits); -1 for any other config.
+// (FP32 5, BF16x3 2; for config 12 also the same result bits); -1 for any
+// other config.
 constexpr int whole_tile_twin(int form, int config) {
-    if (form == kFormBf16x3) return (config == 9 || config == 11 || config == 12) ? 2 : -1;
+    if (form == kFormBf16x3)
+        return (config == 9 || config == 11 || config == 12 || config == 13) ? 2 : -1;
     return (config == 9 || config == 11) ? 5 : -1;
 }
 
@@ -147,10 +149,12 @@ int bf16x3_config_count();
 const ConfigInfo& bf16x3_config(int id);
 hipError_t launch_bf16x3(int id, const Problem& p, hipStream_t stream,
                          const Arena* arena = nullptr);
-// BF16x3 config 12 (packed_x3.hip): split once into packed planes, then the
-// packed 256x256 tile; runs config 2 (same bits) where the planes cannot be
-// had (capture, arena, size).
-hipError_t launch_x3_packed(const Problem& p, hipStream_t stream, const Arena* arena);
+// BF16x3 configs 12 and 13 (packed_x3.hip): split once into packed planes,
+// then the packed 256x256 tile, whole (12) or with config 11's stream-K tail
+// (13); where the planes cannot be had (capture, arena, size) config 2 or 11
+// runs instead, with the same bits.
+hipError_t launch_x3_packed(const Problem& p, hipStream_t stream, const Arena* arena,
+                            bool streamk);
 // Packed-plane rate of config 12's k loop relative to config 2's (GEMM part,
 // 8192^3: 527 against 439 TFLOP/s) and the pack's read + write rate in bytes
 // per microsecond, for the cost model.

@@ -150,13 +150,21 @@ constexpr int kPkNt = 512, kPkTm = 128, kPkTn = 64, kPkMb = 8, kPkNb = 4;
 constexpr int kPkStage = 2 * kPanelTileBytes;
 constexpr int kPkLds = 2 * kPkStage;
 static_assert(epilogue_lds_floats(8, kPkTm, 16) * 4 <= kPkLds, "epilogue LDS");
+// A cut tile's partials: 32 float4 accumulators x 512 threads per piece, in
+// the layout of config 2's split pieces (tile_x3.hpp), so a stream-K tile is
+// summed to the same bits as config 11's.
+constexpr int kPkAcc = kPkMb * kPkNb;
+constexpr int kPkSplitBytes = kPkAcc * kPkNt * 16;
+static_assert(kPkSplitBytes == (int)kX3TileSplitBytes, "config 2's split layout");
 
-__global__ __launch_bounds__(kPkNt) __attribute__((amdgpu_waves_per_eu(1)))
-void k_sgemm_bf16x3_packed(Problem P, const char* __restrict__ pa, const char* __restrict__ pb,
-                           int ktiles) {
-    __shared__ __attribute__((aligned(16))) char smem[kPkLds];
-    int tm, tn;
-    tile_coords(xcd_remap(blockIdx.x, gridDim.x), P.tiles_m, P.tiles_n, 8, tm, tn);
+typedef f32x4 PkAcc[kPkMb][kPkNb];
+
+// Accumulates k-tiles [kt0, kt1) of tile (tm, tn) into acc (zeroed here).
+// All threads of the workgroup call it; on return the LDS is free (a barrier
+// ends every k-tile).
+__device__ inline void packed_accumulate(int tm, int tn, const char* __restrict__ pa,
+                                         const char* __restrict__ pb, int ktiles, int kt0,
+                                         int kt1, char* smem, PkAcc& acc) {
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave % 2, wn = wave / 2;
     const int dw = __builtin_amdgcn_readfirstlane(wave);
@@ -172,20 +180,19 @@ void k_sgemm_bf16x3_packed(Problem P, const char* __restrict__ pa, const char* _
                               kt * kPanelTileBytes + (cbase + r) * 1024 + lane * 16);
     };
     auto rd = [&](const char* p) { return *reinterpret_cast<const bf16x8_t*>(p + lane * 16); };
-    f32x4 acc[kPkMb][kPkNb];
 #pragma unroll
     for (int i = 0; i < kPkMb; ++i)
 #pragma unroll
         for (int j = 0; j < kPkNb; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    if (ktiles > 0) {
-        dma(0, 0);
+    if (kt0 < kt1) {
+        dma(kt0 & 1, kt0);
         wait_vmem();
         __syncthreads();
     }
-    for (int kt = 0; kt < ktiles; ++kt) {
+    for (int kt = kt0; kt < kt1; ++kt) {
         const char* As = smem + (kt & 1) * kPkStage;
         const char* Bs = As + kPanelTileBytes;
-        const bool more = kt + 1 < ktiles;
+        const bool more = kt + 1 < kt1;
         bf16x8_t bh[kPkNb], bl[kPkNb], ah[2], al[2];
 #pragma unroll
         for (int j = 0; j < kPkNb; ++j) {
@@ -216,7 +223,13 @@ void k_sgemm_bf16x3_packed(Problem P, const char* __restrict__ pa, const char* _
         wait_vmem();
         __syncthreads();
     }
-    // Epilogue through LDS, as the other BF16x3 tiles (epilogue_block).
+}
+
+// Epilogue through LDS, as the other BF16x3 tiles (epilogue_block).  The LDS
+// must be free; its first kEpilogueLdsSkip floats are left alone.
+__device__ inline void packed_epilogue(const Problem& P, int tm, int tn, char* smem, PkAcc& acc) {
+    const int wave = threadIdx.x >> 6;
+    const int wm = wave % 2, wn = wave / 2;
     const int m0 = tm * 256, n0 = tn * 256;
     float* elds = reinterpret_cast<float*>(smem) + kEpilogueLdsSkip + wave * 16 * (kPkTm + 4);
 #pragma unroll
@@ -227,33 +240,138 @@ void k_sgemm_bf16x3_packed(Problem P, const char* __restrict__ pa, const char* _
             [&](int, const float* cc, int valid) { return load4(cc, valid); });
 }
 
-}  // namespace
+__global__ __launch_bounds__(kPkNt) __attribute__((amdgpu_waves_per_eu(1)))
+void k_sgemm_bf16x3_packed(Problem P, const char* __restrict__ pa, const char* __restrict__ pb,
+                           int ktiles) {
+    __shared__ __attribute__((aligned(16))) char smem[kPkLds];
+    int tm, tn;
+    tile_coords(xcd_remap(blockIdx.x, gridDim.x), P.tiles_m, P.tiles_n, 8, tm, tn);
+    PkAcc acc;
+    packed_accumulate(tm, tn, pa, pb, ktiles, 0, ktiles, smem, acc);
+    packed_epilogue(P, tm, tn, smem, acc);
+}
 
-hipError_t launch_x3_packed(const Problem& p, hipStream_t s, const Arena* arena) {
+// Config 13: the packed tile with config 11's stream-K tail.  Workgroups
+// [0, sk.dp_tiles) compute whole tiles; the rest run the k-tile ranges of
+// streamk_pieces over the remaining tiles.  A cut tile's pieces store their
+// partials in config 2's split layout and the last to arrive adds them in k
+// order, as tile_x3.hpp does, so the result is config 11's bits.
+__global__ __launch_bounds__(kPkNt) __attribute__((amdgpu_waves_per_eu(1)))
+void k_sgemm_bf16x3_packed_sk(Problem P, SplitK sk, const char* __restrict__ pa,
+                              const char* __restrict__ pb, int ktiles) {
+    __shared__ __attribute__((aligned(16))) char smem[kPkLds];
+    int tm, tn;
+    const int b = blockIdx.x;
+    if (b < sk.dp_tiles) {
+        tile_coords(xcd_remap(b, sk.dp_tiles), P.tiles_m, P.tiles_n, 8, tm, tn);
+        PkAcc acc;
+        packed_accumulate(tm, tn, pa, pb, ktiles, 0, ktiles, smem, acc);
+        packed_epilogue(P, tm, tn, smem, acc);
+        return;
+    }
+    streamk_pieces(P, sk, b, kPk, [&](int tile, const KPiece& pc) {
+        tile_coords(tile, P.tiles_m, P.tiles_n, 8, tm, tn);
+        PkAcc acc;
+        packed_accumulate(tm, tn, pa, pb, ktiles, pc.kbeg / kPk, div_up(pc.kend, kPk), smem, acc);
+        if (pc.n > 1) {
+            const int t = threadIdx.x;
+            const __amdgpu_buffer_rsrc_t part =
+                make_rsrc(sk.partials + (size_t)pc.region * pc.stride * kPkAcc * kPkNt * 4,
+                          (long long)pc.stride * kPkSplitBytes);
+            auto off = [&](int i, int j) { return ((i * kPkNb + j) * kPkNt + t) * 16; };
+#pragma unroll
+            for (int i = 0; i < kPkMb; ++i)
+#pragma unroll
+                for (int j = 0; j < kPkNb; ++j)
+                    store_agent(part, pc.q * kPkSplitBytes + off(i, j), acc[i][j]);
+            if (!splitk_arrive(sk, pc.region, reinterpret_cast<int*>(smem), pc.n)) return;
+            // This workgroup's own piece is re-read like the others (the same
+            // bits as its registers), so the sum runs in k order.
+#pragma unroll
+            for (int i = 0; i < kPkMb; ++i)
+#pragma unroll
+                for (int j = 0; j < kPkNb; ++j) acc[i][j] = load_agent(part, off(i, j));
+            for (int q = 1; q < pc.n; ++q) {
+#pragma unroll
+                for (int i = 0; i < kPkMb; ++i)
+#pragma unroll
+                    for (int j = 0; j < kPkNb; ++j)
+                        acc[i][j] += load_agent(part, q * kPkSplitBytes + off(i, j));
+            }
+        }
+        packed_epilogue(P, tm, tn, smem, acc);
+    });
+}
+
+// Workgroups of the stream-K kernel resident at once (one per CU).
+long long packed_resident_workgroups() {
+    static const long long n = [] {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sgemm_bf16x3_packed_sk, kPkNt,
+                                                         0) != hipSuccess ||
+            per_cu < 1) {
+            (void)hipGetLastError();
+            per_cu = 1;
+        }
+        return (long long)per_cu * device_cu_count();
+    }();
+    return n;
+}
+
+// The packed planes of p in the stream's workspace, launched; false (nothing
+// launched) where they cannot be had.
+bool launch_packs(const Problem& p, hipStream_t s, const Arena* arena, const char** pa,
+                  const char** pb) {
     const int ktiles = div_up(p.k, kPk);
     const size_t bytes_a = packed_bytes(p.m, p.k), bytes_b = packed_bytes(p.n, p.k);
     const long long tiles = (long long)div_up(p.m, 256) * div_up(p.n, 256);
+    if (arena != nullptr || tiles <= 0 || tiles > 0x7fffffffLL ||
+        (long long)ktiles * kPanelTileBytes > 0x7fffffffLL || bytes_a + bytes_b > kPackCapBytes)
+        return false;
     void* ws = nullptr;
-    hipError_t e = hipErrorOutOfMemory;
-    if (arena == nullptr && tiles > 0 && tiles <= 0x7fffffffLL &&
-        (long long)ktiles * kPanelTileBytes <= 0x7fffffffLL &&
-        bytes_a + bytes_b <= kPackCapBytes)
-        e = pack_workspace(s, bytes_a + bytes_b, &ws);
-    // No packed planes (see the file comment): config 2, the same bits.
-    if (e != hipSuccess) return launch_bf16x3(2, p, s, arena);
-    char* pa = static_cast<char*>(ws);
-    char* pb = pa + bytes_a;
+    if (pack_workspace(s, bytes_a + bytes_b, &ws) != hipSuccess) return false;
+    char* a = static_cast<char*>(ws);
+    char* b = a + bytes_a;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_x3_pack_rows, dim3(ktiles, div_up(p.m, kPanel)), dim3(256), 0, s, p.A,
+                       p.m, p.k, p.lda, a, ktiles);
+    hipLaunchKernelGGL(k_x3_pack_cols, dim3(ktiles, div_up(p.n, kPanel)), dim3(256), 0, s, p.B,
+                       p.n, p.k, p.ldb, b, ktiles);
+    *pa = a;
+    *pb = b;
+    return true;
+}
+
+}  // namespace
+
+hipError_t launch_x3_packed(const Problem& p, hipStream_t s, const Arena* arena, bool streamk) {
     Problem q = p;
     q.tiles_m = div_up(p.m, 256);
     q.tiles_n = div_up(p.n, 256);
     q.tile_begin = 0;
-    (void)hipGetLastError();
-    hipLaunchKernelGGL(k_x3_pack_rows, dim3(ktiles, div_up(p.m, kPanel)), dim3(256), 0, s, p.A,
-                       p.m, p.k, p.lda, pa, ktiles);
-    hipLaunchKernelGGL(k_x3_pack_cols, dim3(ktiles, div_up(p.n, kPanel)), dim3(256), 0, s, p.B,
-                       p.n, p.k, p.ldb, pb, ktiles);
-    hipLaunchKernelGGL(k_sgemm_bf16x3_packed, dim3((unsigned)tiles), dim3(kPkNt), 0, s, q,
-                       (const char*)pa, (const char*)pb, ktiles);
+    const long long tiles = (long long)q.tiles_m * q.tiles_n;
+    if (tiles <= 0) return hipSuccess;
+    // The stream-K plan first: its workspace (partials, counters) may fail
+    // to grow under capture, which streamk_plan turns into whole tiles.
+    SplitK sk{nullptr, nullptr, 1, p.k, 0};
+    long long nwg = 0;
+    if (streamk && arena == nullptr) {
+        const hipError_t e = streamk_plan(s, tiles, packed_resident_workgroups(), p.k, kPk,
+                                          kPkSplitBytes, kX3StreamKPieceIters, &sk, &nwg);
+        if (e != hipSuccess) return e;
+    }
+    const char *pa = nullptr, *pb = nullptr;
+    // No packed planes (see the file comment): config 2 or 11, the same bits.
+    if (!launch_packs(p, s, arena, &pa, &pb)) return launch_bf16x3(streamk ? 11 : 2, p, s, arena);
+    const int ktiles = div_up(p.k, kPk);
+    if (nwg > 0) {
+        sk.dp_tiles = (int)(tiles - tiles % packed_resident_workgroups());
+        hipLaunchKernelGGL(k_sgemm_bf16x3_packed_sk, dim3((unsigned)(sk.dp_tiles + nwg)),
+                           dim3(kPkNt), 0, s, q, sk, pa, pb, ktiles);
+    } else {
+        hipLaunchKernelGGL(k_sgemm_bf16x3_packed, dim3((unsigned)tiles), dim3(kPkNt), 0, s, q, pa,
+                           pb, ktiles);
+    }
     return hipGetLastError();
 }
 

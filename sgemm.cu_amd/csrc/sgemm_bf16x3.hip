@@ -62,7 +62,8 @@ void k_sgemm_bf16x3_sk(Problem P, SplitK sk) {
 // (shorter splits of this tile lose to the 64x64 one).  Id 11 runs the tile
 // of id 2 with stream-K over the tiles of a partial last round (splitk = 3).
 // Id 12 splits the operands once into packed planes and runs a 256x256 tile
-// on them (packed_x3.hip), with id 2's result bits.
+// on them (packed_x3.hip), with id 2's result bits; id 13 adds id 11's
+// stream-K tail to it, with id 11's bits.
 static const ConfigInfo kBf16x3Configs[] = {
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m16_dual"},
     {64, 64, 64, 2, 2, "bf16x3_64x64x64_w2x2_m16_dual"},
@@ -77,6 +78,7 @@ static const ConfigInfo kBf16x3Configs[] = {
     {128, 128, 32, 2, 2, "bf16x3_128x128x32_w2x2_m16_single_2cu_splitk", 1},
     {256, 256, 32, 2, 4, "bf16x3_256x256x32_w2x4_m16_single_streamk", 3},
     {256, 256, 32, 2, 4, "bf16x3_256x256x32_w2x4_m16_single_packed"},
+    {256, 256, 32, 2, 4, "bf16x3_256x256x32_w2x4_m16_single_packed_streamk", 3},
 };
 
 int bf16x3_config_count() { return (int)(sizeof(kBf16x3Configs) / sizeof(kBf16x3Configs[0])); }
@@ -218,7 +220,8 @@ hipError_t launch_bf16x3(int id, const Problem& p, hipStream_t s, const Arena* a
         case 9: return launch_x3_tail<X3_256x256_w8>(p, s, arena);
         case 10: return launch_x3<X3_128x128_2cu, true, kX3WideSplitKtiles>(p, s, arena);
         case 11: return launch_x3_streamk<X3_256x256_w8>(p, s, arena);
-        case 12: return launch_x3_packed(p, s, arena);
+        case 12: return launch_x3_packed(p, s, arena, false);
+        case 13: return launch_x3_packed(p, s, arena, true);
         default: break;
     }
     hipError_t e = hipErrorInvalidValue;

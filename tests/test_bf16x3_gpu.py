@@ -22,8 +22,9 @@ from gpu_util import (Case, assert_ok, check, compact, download, log, padding_un
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gemm_cases.json")
-N_X3_CONFIGS = 13
+N_X3_CONFIGS = 14
 X3_PACKED_CONFIG = 12
+X3_PACKED_STREAMK_CONFIG = 13
 
 
 def test_config_table():
@@ -558,3 +559,49 @@ def test_default_pick_packed_and_noalloc_same_bits():
     assert np.array_equal(download(dC), default)
     sa.workspace_release(None)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("m,n,k", [(3000, 3100, 1536), (4600, 4500, 1000), (2600, 2700, 2049)])
+def test_packed_streamk_matches_config11_bit_for_bit(m, n, k):
+    """Config 13 runs the packed tile with config 11's stream-K plan: the
+    tiles of the partial round are cut into k-ranges, and a cut tile's pieces
+    are stored in config 2's split layout and summed in k order, so the bits
+    equal config 11's; also against the FP64 oracle on sampled entries."""
+    c = Case(m, n, k, kind="wide", seed=m + n + k, alpha=1.5, beta=0.5)
+    a = run_form(c, sa.FORM_BF16X3, X3_STREAMK_CONFIG)
+    b = run_form(c, sa.FORM_BF16X3, X3_PACKED_STREAMK_CONFIG)
+    assert np.array_equal(a, b), (m, n, k)
+    _sampled_check_x3(c, b, 2048, m)
+
+
+def test_packed_streamk_capture_fallback_and_noalloc():
+    """Captured on a fresh stream (no workspace), config 13 allocates nothing
+    and runs config 11's tile from A and B (or whole tiles, if its partials
+    cannot grow either), and the no-allocation entry point runs config 11
+    with its static partials: every result equals an eager config-13 call
+    whose plan has the same cut, or config 2's whole tiles."""
+    import torch
+    m, n, k = 3000, 3100, 1536
+    c = Case(m, n, k, seed=88, alpha=1.0, beta=0.0)
+    dA, dB, dC = c.device()
+    sa.run_config(1, X3_PACKED_STREAMK_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m)
+    eager = download(dC).copy()
+    sa.run_config(1, 2, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m)
+    whole = download(dC).copy()
+    fresh = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=fresh):
+        sa.run_config(1, X3_PACKED_STREAMK_CONFIG, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m,
+                      torch.cuda.current_stream())
+    for rep in range(2):
+        dC.fill_(-2.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(download(dC), whole), rep
+    dC.fill_(7.0)
+    assert sa.sgemm_ex_noalloc(sa.FORM_BF16X3, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m) == 0
+    got = download(dC)
+    cfg = sa.launch_config(sa.FORM_BF16X3, m, n, k, dA, m, dB, k)
+    if cfg in (X3_STREAMK_CONFIG, X3_PACKED_STREAMK_CONFIG):
+        assert np.array_equal(got, eager)
