@@ -740,7 +740,9 @@ def main():
         ms = time_form(plain, 1 if args.x3_only else args.steps,
                        0 if args.x3_only else args.warmup, stream)
     tflops = flops / (ms * 1e-3) / 1e12
-    forms = {"f32": {"tflops": tflops, "ms": ms, "sclk_sysfs": clk.summary()}}
+    forms = {"f32": {"tflops": tflops, "ms": ms,
+                     "config": sa.launch_config(sa.FORM_F32, n, n, n, A, n, B, n),
+                     "sclk_sysfs": clk.summary()}}
     # 512 MiB written between calls: more than the 8 x 4 MB L2 plus the 256 MB
     # Infinity Cache, so every flushed call starts with its operands in HBM
     flush_buf = torch.empty(128 << 20, device=dev, dtype=torch.float32)
@@ -759,6 +761,7 @@ def main():
             ms3 = time_form(split, args.steps, args.warmup, stream)
         t3 = flops / (ms3 * 1e-3) / 1e12
         forms["bf16x3"] = {"tflops": t3, "ms": ms3, "speedup_vs_f32": t3 / tflops,
+                           "config": sa.launch_config(sa.FORM_BF16X3, n, n, n, A, n, B, n),
                            "mfma_tflops_issued": 3 * t3, "sclk_sysfs": clk3.summary(),
                            "per_call": per_call(split, min(args.steps, 20), stream),
                            "per_call_flushed": per_call(split, min(args.steps, 10), stream,
