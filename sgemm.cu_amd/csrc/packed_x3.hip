@@ -25,6 +25,8 @@
 // under capture whose pack workspace would have to grow, the no-allocation
 // entry point, a plane past 32-bit offsets or the size cap, a failed
 // allocation -- the launch runs config 2 instead, with the same result.
+#include <mutex>
+
 #include "tile_f32.hpp"
 #include "tile_x3.hpp"
 
@@ -361,6 +363,11 @@ hipError_t launch_x3_packed(const Problem& p, hipStream_t s, const Arena* arena,
         if (e != hipSuccess) return e;
     }
     const char *pa = nullptr, *pb = nullptr;
+    // The packs and the tile go onto the stream together: two host threads
+    // calling on one stream must not interleave one call's packs between
+    // the other's packs and tile, since both use the stream's planes.
+    static std::mutex enqueue_mu;
+    std::lock_guard<std::mutex> lock(enqueue_mu);
     // No packed planes (see the file comment): config 2 or 11, the same bits.
     if (!launch_packs(p, s, arena, &pa, &pb)) return launch_bf16x3(streamk ? 11 : 2, p, s, arena);
     const int ktiles = div_up(p.k, kPk);
