@@ -1,7 +1,8 @@
-// packed_x3.hip -- BF16x3 config 12: the operands are split once, in a pass of
-// their own, into hi/lo planes laid out the way the MFMAs read them, and a
-// 256x256 tile multiplies those planes straight out of LDS.  No reference
-// counterpart (the reference is FP32 FFMA only).
+// packed_x3.hip -- BF16x3 configs 12 and 13: the operands are split once, in a
+// pass of their own, into hi/lo planes laid out the way the MFMAs read them,
+// and a 256x256 tile multiplies those planes straight out of LDS (config 13
+// adds config 11's stream-K tail).  No reference counterpart (the reference
+// is FP32 FFMA only).
 //
 // Why.  The in-register split of the other configs (tile_x3.hpp) splits every
 // element of A once per workgroup that loads it -- n / 256 times over the
@@ -21,10 +22,11 @@
 //
 // Results.  The split is split4 of tile_x3.hpp and every accumulator takes
 // al*bh, ah*bl, ah*bh per k-tile in k order, as config 2 does, so config 12
-// gives config 2's bits.  Where the packed planes cannot be had -- a stream
-// under capture whose pack workspace would have to grow, the no-allocation
-// entry point, a plane past 32-bit offsets or the size cap, a failed
-// allocation -- the launch runs config 2 instead, with the same result.
+// gives config 2's bits and config 13 config 11's.  Where the packed planes
+// cannot be had -- a stream under capture whose pack workspace would have to
+// grow, the no-allocation entry point, a plane past 32-bit offsets or the
+// size cap, a failed allocation -- the launch runs config 2 (or 11) instead,
+// with the same result.
 #include <mutex>
 
 #include "tile_f32.hpp"
