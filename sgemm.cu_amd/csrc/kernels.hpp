@@ -156,10 +156,11 @@ hipError_t launch_bf16x3(int id, const Problem& p, hipStream_t stream,
 hipError_t launch_x3_packed(const Problem& p, hipStream_t stream, const Arena* arena,
                             bool streamk);
 // Packed-plane rate of config 12's k loop relative to config 2's (GEMM part,
-// 8192^3: 527 against 439 TFLOP/s) and the pack's read + write rate in bytes
-// per microsecond, for the cost model.
-constexpr double kX3PackedRate = 1.20;
-constexpr double kX3PackBytesPerUs = 4.8e6;
+// 8192^3 on one box: 526-534 against 433-442 TFLOP/s) and the pack's read +
+// write rate in bytes per microsecond (4.9-5.0 TB/s at 8192^2), for the cost
+// model.
+constexpr double kX3PackedRate = 1.22;
+constexpr double kX3PackBytesPerUs = 5.0e6;
 
 // Naive one-thread-per-output kernel (sgemm_f32.hip).
 hipError_t launch_basic(const Problem& p, hipStream_t stream);
