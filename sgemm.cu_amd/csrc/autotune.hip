@@ -97,6 +97,24 @@ int heuristic(int form, int grouped, int m, int n, int k, bool aligned) {
         const int short_k = aligned && t128 >= 1024 ? 160 : 256;
         if (k <= short_k && t128 >= 512) return 14;
         if (aligned) {
+            // Config 15 (256x256x32, one wave per SIMD and one workgroup per
+            // CU; config 5's result bits): 8192^3 145.4 -> 150.1,
+            // 8192x8192x16384 151.2 TFLOP/s against config 9 on one box.  Its
+            // tile costs about 25 us more per round (prologue and epilogue of
+            // a whole chip at once), so below k = 2048 config 9 is as fast or
+            // faster (8192x8192x2048 142 vs 143, x512 117 vs 130).  It runs
+            // whole tiles only, so it is taken where its modelled rate --
+            // 149.5 x the filled share of its last round x the useful share
+            // of its padded tiles -- beats config 9's 145 x the useful share
+            // of 128x128 tiles (config 9's tail split fills its last round).
+            if (t128 >= 1024 && k % 32 == 0 && k >= 2048) {
+                const double cus = device_cu_count();
+                const double t256 = (double)div_up(m, 256) * div_up(n, 256);
+                const double c = t256 / cus;
+                const double e15 = 149.5 * (c / std::ceil(c)) * ((double)m * n / (t256 * 65536.0));
+                const double e9 = 145.0 * ((double)m * n / ((double)t128 * 16384.0));
+                if (e15 > e9) return 15;
+            }
             if (t128 >= 1024) return 9;
             // Short of one round of 1024 direct-loaded 128x128 tiles, with k
             // >= 1024 and from 180 tiles: every tile of a config takes the

@@ -132,9 +132,11 @@ __device__ inline f32x4 bload4(__amdgpu_buffer_rsrc_t r, int off) {
 // data lands at lds + 4 * i floats (buffer_load_dwordx4 with the lds bit;
 // `lds` must be the same for every lane).  Out-of-range dwords write 0.
 // Completion is counted by vmcnt, not lgkmcnt: see wait_vmem.
-__device__ inline void buffer_load_lds16(__amdgpu_buffer_rsrc_t r, float* lds, int off) {
+// soff: a wave-uniform byte offset added in the instruction's scalar offset.
+__device__ inline void buffer_load_lds16(__amdgpu_buffer_rsrc_t r, float* lds, int off,
+                                         int soff = 0) {
     __builtin_amdgcn_raw_ptr_buffer_load_lds(
-        r, (__attribute__((address_space(3))) void*)lds, 16, off, 0, 0, 0);
+        r, (__attribute__((address_space(3))) void*)lds, 16, off, soff, 0, 0);
 }
 
 // s_waitcnt vmcnt(0): every vector memory load of this wave has completed,

@@ -144,6 +144,13 @@ const ConfigInfo& f32_config(int id);
 // arena: see splitk_workspace (nullptr: the stream's workspace).
 hipError_t launch_f32(int id, const Problem& p, hipStream_t stream, const Arena* arena = nullptr);
 
+// Plain FP32 config 15 (f32_256.hip): the 256x256x32 tile of four waves,
+// one per SIMD, for 16-byte aligned A and B columns and k a multiple of 32
+// (at least 64); config 5's result bits.  launch_f32 runs config 9 where
+// f32_256_supported does not hold.
+bool f32_256_supported(const Problem& p);
+hipError_t launch_f32_256(const Problem& p, hipStream_t stream);
+
 // BF16x3 configs (sgemm_bf16x3.hip).
 int bf16x3_config_count();
 const ConfigInfo& bf16x3_config(int id);

@@ -95,7 +95,8 @@ def test_launch_config_follows_operand_alignment():
     3584^3 (784 tiles, 3.06 per CU, 12.25 64x64 tiles per CU) the 64x64
     tile (config 7); at 3968^3 (961 tiles, 3.75 per CU) config 5.  With odd
     leading dimensions the register-staged tile (768 resident) takes the
-    tail split (config 9) from 700 tiles.  k == 0 and leading dimensions
+    tail split (config 9) from 700 tiles.  Whole rounds of 256x256 tiles with
+    aligned columns and k a multiple of 32 take config 15.  k == 0 and leading dimensions
     past 2^21 run no tile config (-1)."""
     F = sa.FORM_F32
     base = 1 << 20                                # 16-byte aligned fake addresses
@@ -106,7 +107,11 @@ def test_launch_config_follows_operand_alignment():
     assert sa.launch_config(F, 3968, 3968, 3968, base, 3968, base, 3968) == 5
     assert sa.launch_config(F, m, n, k, base, m + 1, base, k) == 9      # odd lda
     assert sa.launch_config(F, m, n, k, base + 4, m, base, k) == 9      # A 4 bytes off
-    assert sa.launch_config(F, 8192, 8192, 8192, base, 8192, base, 8192) == 9
+    # whole rounds of 256x256 tiles with k % 32 == 0: config 15; a partial
+    # last round (6144^3: 2.25 rounds) or odd columns: config 9
+    assert sa.launch_config(F, 8192, 8192, 8192, base, 8192, base, 8192) == 15
+    assert sa.launch_config(F, 8192, 8192, 8192, base, 8193, base, 8192) == 9
+    assert sa.launch_config(F, 6144, 6144, 6144, base, 6144, base, 6144) == 9
     assert sa.launch_config(F, 8192, 8192, 127, base, 8193, base, 127) == 14
     assert sa.launch_config(F, m, n, 0, base, m, base, 1) == -1
     assert sa.launch_config(F, m, n, k, base, (1 << 21) + 4, base, k) == -1
