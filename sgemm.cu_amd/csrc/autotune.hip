@@ -97,23 +97,31 @@ int heuristic(int form, int grouped, int m, int n, int k, bool aligned) {
         const int short_k = aligned && t128 >= 1024 ? 160 : 256;
         if (k <= short_k && t128 >= 512) return 14;
         if (aligned) {
-            // Config 15 (256x256x32, one wave per SIMD and one workgroup per
-            // CU; config 5's result bits): 8192^3 145.4 -> 150.1,
-            // 8192x8192x16384 151.2 TFLOP/s against config 9 on one box.  Its
-            // tile costs about 25 us more per round (prologue and epilogue of
-            // a whole chip at once), so below k = 2048 config 9 is as fast or
-            // faster (8192x8192x2048 142 vs 143, x512 117 vs 130).  It runs
-            // whole tiles only, so it is taken where its modelled rate --
-            // 149.5 x the filled share of its last round x the useful share
-            // of its padded tiles -- beats config 9's 145 x the useful share
-            // of 128x128 tiles (config 9's tail split fills its last round).
+            // Configs 15 and 16 (256x256x32, one wave per SIMD and one
+            // workgroup per CU; 15 gives config 5's bits, 16 adds a stream-K
+            // tail): 8192^3 145.4 -> 150.1, 13312^3 146.3 -> 150.1 (16),
+            // 10240^3 145.8 -> 149.1 (16) TFLOP/s against config 9 on one box.
+            // Modelled times: a round of their tiles costs I k-tiles of
+            // kF256KtileUs each plus kF256RoundUs (prologue and epilogue of
+            // the whole chip at once), config 16's partial round what
+            // streamk_iters charges, and config 9 runs its padded tiles at
+            // 145 TFLOP/s.  Below k = 2048 the round cost makes config 9 as
+            // fast or faster (8192x8192x2048 142 vs 143, x512 117 vs 130); a
+            // tail of a few tiles cut into many pieces loses to it too (9216^3
+            // 144.5 vs 145.6, 7168^3 141.7 vs 144.9).
             if (t128 >= 1024 && k % 32 == 0 && k >= 2048) {
-                const double cus = device_cu_count();
-                const double t256 = (double)div_up(m, 256) * div_up(n, 256);
-                const double c = t256 / cus;
-                const double e15 = 149.5 * (c / std::ceil(c)) * ((double)m * n / (t256 * 65536.0));
-                const double e9 = 145.0 * ((double)m * n / ((double)t128 * 16384.0));
-                if (e15 > e9) return 15;
+                const long long cus = device_cu_count();
+                const long long t256 = (long long)div_up(m, 256) * div_up(n, 256);
+                const long long iters = k / 32, full = t256 / cus, rem = t256 % cus;
+                const double round = (double)iters * kF256KtileUs + kF256RoundUs;
+                const double t15 = (double)(full + (rem > 0 ? 1 : 0)) * round;
+                double t16 = t15;
+                double part = 1.0;
+                if (rem > 0 && streamk_iters(rem, cus, iters, kF256SplitBytes,
+                                             kF256StreamKPieceIters, &part) > 0)
+                    t16 = (double)full * round + part * iters * kF256KtileUs + kF256RoundUs;
+                const double t9 = 2.0 * t128 * 16384.0 * k / 145.0e6;
+                if (std::min(t15, t16) < t9) return t16 < t15 ? 16 : 15;
             }
             if (t128 >= 1024) return 9;
             // Short of one round of 1024 direct-loaded 128x128 tiles, with k

@@ -103,12 +103,13 @@ __device__ __forceinline__ void schedule_groups() {
     }
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
-void k_sgemm_f32_256(Problem P) {
-    __shared__ __attribute__((aligned(16))) float smem[kLdsFloats];
-    int tm, tn;
-    tile_coords(xcd_remap(blockIdx.x, gridDim.x), P.tiles_m, P.tiles_n, kGroupN256, tm, tn);
-    const int m0 = tm * kBM, n0 = tn * kBN;
+// Accumulates k-tiles [kt0, kt1) of tile (m0, n0) into acc, from zero:
+// kt1 - kt0 >= 2, or >= 1 with ONE (stream-K pieces).  Every thread of the
+// workgroup calls it; on return the workgroup may still be reading LDS (the
+// caller's next LDS use follows a barrier).
+template <bool ONE>
+__device__ __forceinline__ void f256_accumulate(const Problem& P, int m0, int n0, int kt0,
+                                                int kt1, float* smem, f32x4 (&acc)[8][8]) {
     const int mrows = min(kBM, P.m - m0), ncols = min(kBN, P.n - n0);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -142,19 +143,27 @@ void k_sgemm_f32_256(Problem P) {
             buffer_load_lds16(rb, st + kAStage + (64 * wave + 8 * r) * kBK, b_off[r], kb);
     };
 
-    f32x4 acc[8][8];
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
         for (int j = 0; j < 8; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    const int KT = K / kBK;  // >= 2
+    // Stage (t & 1) holds k-tile kt0 + t.
+    const int KT = kt1 - kt0;
     Half f0, f1;
-    dma(smem, 0);
-    dma(smem + kStage, 1);
+    dma(smem, kt0);
+    if (!ONE || KT > 1) dma(smem + kStage, kt0 + 1);
     wait_vmem();
     __syncthreads();
     read_half(f0, smem, 0, wm, wn, d, g);
+    if constexpr (ONE) {
+        if (KT == 1) {
+            read_half(f1, smem, 1, wm, wn, d, g);
+            mma_half(acc, f0);
+            mma_half(acc, f1);
+            return;
+        }
+    }
 
     // This wave's direct loads of the next k-tile have reached LDS and its
     // operand reads of the current one are done; then every wave's are.
@@ -171,7 +180,7 @@ void k_sgemm_f32_256(Problem P) {
         schedule_groups<16, 0, 1>();
         __builtin_amdgcn_sched_barrier(0);
         sync();
-        dma(smem + (t & 1) * kStage, t + 2);
+        dma(smem + (t & 1) * kStage, kt0 + t + 2);
         mma_half<0, 2>(acc, f1);
         schedule_groups<8, 2, 0>();
         __builtin_amdgcn_sched_barrier(0);
@@ -191,12 +200,20 @@ void k_sgemm_f32_256(Problem P) {
     read_half(f1, smem + (t & 1) * kStage, 1, wm, wn, d, g);
     mma_half(acc, f0);
     mma_half(acc, f1);
+}
 
-    // Epilogue, per 16-column block j: each wave writes its 128 x 16 results
-    // to its own LDS area as [column][row] (rows of block 4h + e's register
-    // group q are rows 64h + 16g + 4q + e, so the four blocks of one h give
-    // four consecutive rows) and reads back two whole columns per pass.
-    __syncthreads();  // every wave is past its last operand read
+// Epilogue of tile (m0, n0), per 16-column block j: each wave writes its
+// 128 x 16 results to its own LDS area as [column][row] (rows of block 4h +
+// e's register group q are rows 64h + 16g + 4q + e, so the four blocks of
+// one h give four consecutive rows) and reads back two whole columns per
+// pass.  Starts with a barrier: every wave is past its last operand read.
+__device__ __forceinline__ void f256_epilogue(const Problem& P, int m0, int n0, float* smem,
+                                              const f32x4 (&acc)[8][8]) {
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = wave & 1, wn = wave >> 1;
+    const int d = lane & 15, g = lane >> 4;
     float* wlds = smem + wave * (16 * kEpiStride);
     const int c_in = lane >> 5, r4 = 4 * (lane & 31);
     const int row = m0 + 128 * wm + r4;
@@ -241,6 +258,106 @@ void k_sgemm_f32_256(Problem P) {
     }
 }
 
+// One piece's accumulator (i, j) of thread t at byte 16 t + 4096 (8 i + j):
+// the per-thread part is the vector offset, the rest the scalar offset, so
+// the offsets take no registers of their own.
+__device__ __forceinline__ void store_piece(__amdgpu_buffer_rsrc_t r, int voff, int soff,
+                                            f32x4 v) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, voff, soff,
+                                           kPolicyAgent);
+}
+__device__ __forceinline__ f32x4 load_piece(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(f32x4,
+                              __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, kPolicyAgent));
+}
+
+// Config 15: whole tiles.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_sgemm_f32_256(Problem P) {
+    __shared__ __attribute__((aligned(16))) float smem[kLdsFloats];
+    int tm, tn;
+    tile_coords(xcd_remap(blockIdx.x, gridDim.x), P.tiles_m, P.tiles_n, kGroupN256, tm, tn);
+    f32x4 acc[8][8];
+    f256_accumulate<false>(P, tm * kBM, tn * kBN, 0, P.k / kBK, smem, acc);
+    f256_epilogue(P, tm * kBM, tn * kBN, smem, acc);
+}
+
+// Config 16: config 15 with stream-K over the tiles of a partial last round.
+// Workgroups [0, sk.dp_tiles) compute whole tiles; the rest run the k-tile
+// ranges of streamk_pieces over the remaining tiles.  A cut tile's pieces
+// store their accumulators (in lane order, 256 KB per piece) and the last to
+// arrive adds them in k order, so the result does not depend on which
+// piece finishes last.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_sgemm_f32_256_sk(Problem P, SplitK sk) {
+    __shared__ __attribute__((aligned(16))) float smem[kLdsFloats];
+    int tm, tn;
+    const int b = blockIdx.x;
+    if (b < sk.dp_tiles) {
+        tile_coords(xcd_remap(b, sk.dp_tiles), P.tiles_m, P.tiles_n, kGroupN256, tm, tn);
+        f32x4 acc[8][8];
+        f256_accumulate<false>(P, tm * kBM, tn * kBN, 0, P.k / kBK, smem, acc);
+        f256_epilogue(P, tm * kBM, tn * kBN, smem, acc);
+        return;
+    }
+    streamk_pieces(P, sk, b, kBK, [&](int tile, const KPiece& pc) {
+        tile_coords(tile, P.tiles_m, P.tiles_n, kGroupN256, tm, tn);
+        f32x4 acc[8][8];
+        f256_accumulate<true>(P, tm * kBM, tn * kBN, pc.kbeg / kBK, pc.kend / kBK, smem, acc);
+        if (pc.n > 1) {
+            const int t = threadIdx.x;
+            const __amdgpu_buffer_rsrc_t part =
+                make_rsrc(sk.partials + (size_t)pc.region * pc.stride * (kF256SplitBytes / 4),
+                          (long long)pc.stride * kF256SplitBytes);
+            const int voff = 16 * t;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    store_piece(part, voff, pc.q * kF256SplitBytes + 4096 * (8 * i + j),
+                                acc[i][j]);
+            __syncthreads();  // every wave is past its operand reads: LDS is free
+            if (!splitk_arrive(sk, pc.region, reinterpret_cast<int*>(smem), pc.n)) return;
+            // This workgroup's own piece is re-read like the others (the same
+            // bits as its registers), so the sum runs in k order.  One row of
+            // eight blocks at a time (the scheduling barriers keep the loads
+            // of a row from being hoisted above the previous row's adds, which
+            // would need registers for all 64 at once).
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) acc[i][j] = load_piece(part, voff, 4096 * (8 * i + j));
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            for (int q = 1; q < pc.n; ++q) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        acc[i][j] += load_piece(part, voff, q * kF256SplitBytes + 4096 * (8 * i + j));
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+        }
+        f256_epilogue(P, tm * kBM, tn * kBN, smem, acc);
+    });
+}
+
+// Workgroups of the stream-K kernel resident at once (one per CU).
+long long f256_resident_workgroups() {
+    static const long long n = [] {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sgemm_f32_256_sk, 256, 0) !=
+                hipSuccess ||
+            per_cu < 1) {
+            (void)hipGetLastError();
+            per_cu = 1;
+        }
+        return (long long)per_cu * device_cu_count();
+    }();
+    return n;
+}
+
 }  // namespace
 
 bool f32_256_supported(const Problem& p) {
@@ -252,7 +369,7 @@ bool f32_256_supported(const Problem& p) {
            a_bytes < (1LL << 31) && b_bytes < (1LL << 31);
 }
 
-hipError_t launch_f32_256(const Problem& p, hipStream_t s) {
+hipError_t launch_f32_256(const Problem& p, hipStream_t s, bool streamk, const Arena* arena) {
     Problem q = p;
     q.tiles_m = div_up(p.m, kBM);
     q.tiles_n = div_up(p.n, kBN);
@@ -260,8 +377,24 @@ hipError_t launch_f32_256(const Problem& p, hipStream_t s) {
     const long long tiles = (long long)q.tiles_m * q.tiles_n;
     if (tiles <= 0) return hipSuccess;
     if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
+    SplitK sk{nullptr, nullptr, 1, p.k, 0};
+    long long nwg = 0;
+    // The no-allocation entry point (arena) runs the whole tiles: its array
+    // is far smaller than the stream-K partials of this tile.
+    if (streamk && arena == nullptr) {
+        const hipError_t e = streamk_plan(s, tiles, f256_resident_workgroups(), p.k, kBK,
+                                          kF256SplitBytes, kF256StreamKPieceIters, &sk, &nwg,
+                                          nullptr);
+        if (e != hipSuccess) return e;
+    }
     (void)hipGetLastError();  // the status read below must belong to this launch
-    hipLaunchKernelGGL(k_sgemm_f32_256, dim3((unsigned)tiles), dim3(256), 0, s, q);
+    if (nwg > 0) {
+        sk.dp_tiles = (int)(tiles - tiles % f256_resident_workgroups());
+        hipLaunchKernelGGL(k_sgemm_f32_256_sk, dim3((unsigned)(sk.dp_tiles + nwg)), dim3(256), 0,
+                           s, q, sk);
+    } else {
+        hipLaunchKernelGGL(k_sgemm_f32_256, dim3((unsigned)tiles), dim3(256), 0, s, q);
+    }
     return hipGetLastError();
 }
 

@@ -75,6 +75,7 @@ constexpr double kX3StreamKPieceIters = 7.0;
 constexpr int whole_tile_twin(int form, int config) {
     if (form == kFormBf16x3)
         return (config == 9 || config == 11 || config == 12 || config == 13) ? 2 : -1;
+    if (config == 16) return 15;
     return (config == 9 || config == 11) ? 5 : -1;
 }
 
@@ -148,8 +149,22 @@ hipError_t launch_f32(int id, const Problem& p, hipStream_t stream, const Arena*
 // one per SIMD, for 16-byte aligned A and B columns and k a multiple of 32
 // (at least 64); config 5's result bits.  launch_f32 runs config 9 where
 // f32_256_supported does not hold.
+// Config 16 is config 15 with a stream-K tail (streamk): the tiles of a
+// partial last round are spread k-tile by k-tile over the CUs, as config 11
+// does for config 5's tile; with an arena it runs config 15's whole tiles.
 bool f32_256_supported(const Problem& p);
-hipError_t launch_f32_256(const Problem& p, hipStream_t stream);
+hipError_t launch_f32_256(const Problem& p, hipStream_t stream, bool streamk,
+                          const Arena* arena = nullptr);
+// Config 15/16 cost model (8192x8192x{8192, 16384} on one box): one
+// 256x256x32 k-tile of one tile takes kF256KtileUs on its CU (152 TFLOP/s
+// over 256 CUs), a round of tiles kF256RoundUs more; one stream-K piece of
+// config 16 costs kF256StreamKPieceIters k-tiles (storing 256 KB of
+// partials and, for the last piece, reading every piece back; fitted to
+// 9216^3, 10240^3 and 13312^3).
+constexpr double kF256KtileUs = 7.06;
+constexpr double kF256RoundUs = 25.0;
+constexpr double kF256StreamKPieceIters = 4.0;
+constexpr size_t kF256SplitBytes = 64 * 256 * 16;  // accumulators of one piece
 
 // BF16x3 configs (sgemm_bf16x3.hip).
 int bf16x3_config_count();

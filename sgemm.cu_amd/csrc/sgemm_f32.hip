@@ -65,7 +65,8 @@ void k_sgemm_f32_sk(Problem P, SplitK sk) {
 // 5 with stream-K over the tiles of a partial last round (splitk = 3).  Id
 // 12 is a 64x64x64 tile with two k groups of four waves (F32_64x64_kg2).
 // Id 15 is the 256x256x32 tile of f32_256.hip (one wave per SIMD), with id
-// 9 standing in where its preconditions do not hold.
+// 9 standing in where its preconditions do not hold; id 16 is id 15 with
+// stream-K over the tiles of a partial last round.
 static const ConfigInfo kF32Configs[] = {
     {128, 128, 32, 2, 2, "f32_128x128x32_w2x2"},
     {64, 64, 32, 2, 2, "f32_64x64x32_w2x2"},
@@ -83,6 +84,7 @@ static const ConfigInfo kF32Configs[] = {
     {128, 128, 32, 2, 2, "f32_128x128x32_w2x2_cprefetch"},
     {128, 64, 16, 2, 2, "f32_128x64x16_w2x2_cprefetch"},
     {256, 256, 32, 2, 2, "f32_256x256x32_w2x2_1wpe"},
+    {256, 256, 32, 2, 2, "f32_256x256x32_w2x2_1wpe_streamk", 3},
 };
 
 int f32_config_count() { return (int)(sizeof(kF32Configs) / sizeof(kF32Configs[0])); }
@@ -244,7 +246,8 @@ hipError_t launch_f32(int id, const Problem& p, hipStream_t s, const Arena* aren
         case 11: return launch_streamk<F32_128x128_k16>(p, s, arena);
         case 12: return launch_tile<F32_64x64_kg2, false>(p, s, arena);
         case 15:
-            if (f32_256_supported(p)) return launch_f32_256(p, s);
+        case 16:
+            if (f32_256_supported(p)) return launch_f32_256(p, s, id == 16, arena);
             return dl ? launch_tail_split<F32_128x128_k16_dl>(p, s, arena)
                       : launch_tail_split<F32_128x128_k16>(p, s, arena);
         default: break;

@@ -96,7 +96,8 @@ def test_launch_config_follows_operand_alignment():
     tile (config 7); at 3968^3 (961 tiles, 3.75 per CU) config 5.  With odd
     leading dimensions the register-staged tile (768 resident) takes the
     tail split (config 9) from 700 tiles.  Whole rounds of 256x256 tiles with
-    aligned columns and k a multiple of 32 take config 15.  k == 0 and leading dimensions
+    aligned columns and k a multiple of 32 take config 15, partial rounds
+    config 16 where the time model says its stream-K tail pays.  k == 0 and leading dimensions
     past 2^21 run no tile config (-1)."""
     F = sa.FORM_F32
     base = 1 << 20                                # 16-byte aligned fake addresses
@@ -112,6 +113,10 @@ def test_launch_config_follows_operand_alignment():
     assert sa.launch_config(F, 8192, 8192, 8192, base, 8192, base, 8192) == 15
     assert sa.launch_config(F, 8192, 8192, 8192, base, 8193, base, 8192) == 9
     assert sa.launch_config(F, 6144, 6144, 6144, base, 6144, base, 6144) == 9
+    # a partial round that stream-K spreads cheaply: config 16 (13312^3: 144
+    # tiles past ten rounds); a few tiles cut into many pieces: config 9
+    assert sa.launch_config(F, 13312, 13312, 13312, base, 13312, base, 13312) == 16
+    assert sa.launch_config(F, 9216, 9216, 9216, base, 9216, base, 9216) == 9
     assert sa.launch_config(F, 8192, 8192, 127, base, 8193, base, 127) == 14
     assert sa.launch_config(F, m, n, 0, base, m, base, 1) == -1
     assert sa.launch_config(F, m, n, k, base, (1 << 21) + 4, base, k) == -1

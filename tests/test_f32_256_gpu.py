@@ -106,3 +106,128 @@ def test_default_entry_point_picks_config15_for_whole_rounds():
         assert sa.launch_config(sa.FORM_F32, m, n, k, base, m, base, k) == 15, (m, n, k)
     assert sa.launch_config(sa.FORM_F32, 8192, 8192, 8192, base, 8193, base, 8192) == 9
     assert sa.launch_config(sa.FORM_F32, 8192, 8192, 8200, base, 8192, base, 8200) == 9
+
+
+# ----------------------------------------------- config 16: stream-K tail
+def _sampled(c, out, n_samples, seed):
+    rng = np.random.default_rng(seed)
+    idx = rng.integers(0, c.m * c.n, size=n_samples)
+    rows, cols = idx % c.m, idx // c.m
+    ref, scale = orc.dgemm_entries(c.k, c.alpha, c.A, c.lda, c.B, c.ldb, c.beta, c.C0, c.ldc,
+                                   rows, cols)
+    got = out[cols * c.ldc + rows].astype(np.float64)
+    err = np.abs(got - ref) / np.maximum(scale, np.finfo(np.float64).tiny)
+    assert err.max() <= orc.fp32_tol_mult(c.k), err.max()
+
+
+@pytest.mark.parametrize("m,n,k", [
+    (1000, 1100, 2048),   # 20 tiles of 256x256, no full round: every tile cut
+    (2816, 2816, 1536),   # 121 tiles
+    (4352, 4352, 1024),   # 289 tiles: one round and 33
+    (300, 260, 96),       # one tile of three k-tiles
+])
+def test_config16_deterministic_and_in_tolerance(m, n, k):
+    """Stream-K over the tiles of a partial round (or all tiles): within the
+    FP32 bound of the FP64 oracle, and bit-identical on repeat with another
+    problem of the same shape in between (partial slots and counters are
+    reused)."""
+    c = Case(m, n, k, ldc=m + 1, seed=3 * m + n + k, alpha=1.25, beta=0.5)
+    first = _run(16, c)
+    if m * n <= (1 << 21):
+        assert_ok(check(c, first, orc.fp32_tol_mult(k)), f"config 16 {(m, n, k)}")
+    else:
+        _sampled(c, first, 4096, k)
+    other = Case(m, n, k, ldc=m + 1, seed=m + n + k + 13, alpha=1.0, beta=0.0)
+    _sampled(other, _run(16, other), 2048, k + 5)
+    again = _run(16, c)
+    assert np.array_equal(first, again), (m, n, k)
+
+
+def test_config16_integer_kat_equals_whole_tiles():
+    """Integer inputs: every partial sum is an exact integer, so the stream-K
+    tail equals config 5's whole tiles bit for bit."""
+    for (m, n, k) in [(2816, 2432, 1536), (4352, 4352, 1024)]:
+        c = Case(m, n, k, kind="int", seed=29, alpha=1.5, beta=0.5)
+        assert np.array_equal(_run(5, c), _run(16, c)), (m, n, k)
+
+
+def test_config16_graph_replays_and_capture_fallback():
+    """Captured after a warm-up call on the stream, config 16 replays
+    bit-identically to its eager result; captured on a fresh stream (no
+    workspace yet) it runs config 15's whole tiles, config 5's bits."""
+    import torch
+    m, n, k = 2816, 2560, 1024
+    c = Case(m, n, k, seed=81, alpha=1.0, beta=0.0)
+    dA, dB, dC = c.device()
+    log(f"config 16 graph m={m} n={n} k={k}")
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        sa.run_config(0, 16, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, s)
+        s.synchronize()
+        eager = download(dC).copy()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            sa.run_config(0, 16, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m,
+                          torch.cuda.current_stream())
+    for rep in range(3):
+        dC.fill_(5.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(download(dC), eager), rep
+    sa.run_config(0, 5, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m)
+    torch.cuda.synchronize()
+    whole = download(dC).copy()
+    assert not np.array_equal(whole, eager)  # the tiles were cut
+    fresh = torch.cuda.Stream()
+    g2 = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g2, stream=fresh):
+        sa.run_config(0, 16, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m,
+                      torch.cuda.current_stream())
+    for rep in range(2):
+        dC.fill_(-2.0)
+        torch.cuda.synchronize()
+        g2.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(download(dC), whole), rep
+
+
+def test_config16_concurrent_streams_match_alone():
+    """Each stream has its own workspace: two config-16 launches of different
+    problems on two streams at once give the bits each gives alone."""
+    import torch
+    shapes = [(2816, 2560, 1536), (2432, 2600, 1024)]
+    cases = [Case(m, n, k, seed=95 + i, alpha=1.0, beta=0.0) for i, (m, n, k) in enumerate(shapes)]
+    devs = [c.device() for c in cases]
+    alone = []
+    for c, (dA, dB, dC) in zip(cases, devs):
+        log(f"config 16 alone m={c.m} n={c.n} k={c.k}")
+        sa.run_config(0, 16, c.m, c.n, c.k, 1.0, dA, c.m, dB, c.k, 0.0, dC, c.m)
+        torch.cuda.synchronize()
+        alone.append(download(dC).copy())
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for rep in range(3):
+        for (_, _, dC) in devs:
+            dC.fill_(3.0)
+        torch.cuda.synchronize()
+        for c, (dA, dB, dC), s in zip(cases, devs, streams):
+            sa.run_config(0, 16, c.m, c.n, c.k, 1.0, dA, c.m, dB, c.k, 0.0, dC, c.m, s)
+        torch.cuda.synchronize()
+        for (_, _, dC), want in zip(devs, alone):
+            assert np.array_equal(download(dC), want), rep
+
+
+def test_config16_noalloc_runs_whole_tiles():
+    """The no-allocation entry point never takes the stream-K workspace:
+    a config-16 pick there runs config 15's whole tiles."""
+    m, n, k = 4352, 4352, 1024
+    c = Case(m, n, k, seed=7, alpha=1.5, beta=0.5)
+    dA, dB, dC = c.device()
+    sa.autotune_set(sa.FORM_F32, m, n, k, 16)
+    try:
+        log(f"config 16 noalloc m={m} n={n} k={k}")
+        sa.sgemm_ex_noalloc(sa.FORM_F32, m, n, k, 1.5, dA, m, dB, k, 0.5, dC, m)
+        got = download(dC)
+    finally:
+        sa.autotune_clear()
+    assert np.array_equal(got, _run(15, c))

@@ -24,7 +24,7 @@ from gpu_util import (Case, assert_ok, check, compact, download, log, padding_un
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gemm_cases.json")
-N_F32_CONFIGS = 16
+N_F32_CONFIGS = 17
 
 
 @pytest.fixture(scope="module")
