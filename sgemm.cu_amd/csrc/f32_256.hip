@@ -31,10 +31,10 @@
 //
 // Preconditions (f32_256_supported): 16-byte aligned A and B columns, k a
 // multiple of 32 and at least 64, byte offsets within a tile's operand
-// panels below 2^31.  Partial tiles in m and n are fine: each
-// operand's buffer resource is bounded by its extent, so no load faults, rows past m and columns past n reach only C
-// entries that are never stored, and the epilogue stores only inside C.
-// Elsewhere config 15 runs config 9.
+// panels below 2^31.  Partial tiles in m and n are fine: each operand's
+// buffer resource is bounded by its extent, so no load faults; rows past m
+// and columns past n reach only C entries that are never stored, and the
+// epilogue stores only inside C.  Elsewhere config 15 runs config 9.
 #include "tile_f32.hpp"
 
 namespace sgemm_amd {
@@ -121,8 +121,11 @@ __device__ __forceinline__ void f256_accumulate(const Problem& P, int m0, int n0
     // 256 m per instruction) and B's columns 64w .. 64w + 63 (eight columns
     // of 32 k per instruction, lane l at position l % 8 of column l / 8,
     // which holds chunk (l % 8) ^ ((n / 2) % 8)).  One buffer per operand
-    // for the whole k loop; the k-tile's byte offset is the instruction's
-    // scalar offset, so a load costs no vector arithmetic.
+    // for the whole k loop.  B's k-tile offset goes in the instruction's
+    // scalar offset (no vector arithmetic): its out-of-range columns are out
+    // of range at any k.  A's goes in the vector offset, which the buffer's
+    // range check covers: rows past m of the last column lie past A's end
+    // and must read as 0, not be fetched.
     const auto ra = make_rsrc(P.A + m0, extent(mrows, K, lda) * 4);
     const auto rb = make_rsrc(P.B + (size_t)n0 * ldb, extent(K, ncols, ldb) * 4);
     int a_off[8], b_off[8];
@@ -136,8 +139,7 @@ __device__ __forceinline__ void f256_accumulate(const Problem& P, int m0, int n0
     auto dma = [&](float* st, int kt) {
         const int ka = kt * kBK * lda * 4, kb = kt * kBK * 4;
 #pragma unroll
-        for (int r = 0; r < 8; ++r)
-            buffer_load_lds16(ra, st + (8 * wave + r) * kBM, a_off[r], ka);
+        for (int r = 0; r < 8; ++r) buffer_load_lds16(ra, st + (8 * wave + r) * kBM, a_off[r] + ka);
 #pragma unroll
         for (int r = 0; r < 8; ++r)
             buffer_load_lds16(rb, st + kAStage + (64 * wave + 8 * r) * kBK, b_off[r], kb);
