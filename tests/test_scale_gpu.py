@@ -147,12 +147,14 @@ def test_serving_bench_config_alone_equals_mix(form):
 
 
 @pytest.mark.parametrize("form,split_cfg", [(sa.FORM_BF16X3, 11), (sa.FORM_BF16X3, 9),
-                                            (sa.FORM_F32, 11), (sa.FORM_F32, 9)])
+                                            (sa.FORM_F32, 11), (sa.FORM_F32, 9),
+                                            (sa.FORM_F32, 16)])
 def test_panels_run_split_picks_as_whole_tiles(form, split_cfg):
-    """With the panel shapes' buckets set to the stream-K (11) or tail-split
-    (9) config, the server runs each panel on the same tile whole (BF16x3
-    config 2, FP32 config 5): the request's result equals one launch of that
-    whole-tile config over the full problem, alone and in a mix."""
+    """With the panel shapes' buckets set to the stream-K (11, FP32 16) or
+    tail-split (9) config, the server runs each panel on the same tile whole
+    (BF16x3 config 2, FP32 config 5 or 15, which gives config 5's bits): the
+    request's result equals one launch of config 2 or 5 over the full
+    problem, alone and in a mix."""
     import torch
     whole = 2 if form == sa.FORM_BF16X3 else 5
     m, n, k = 4096, 5000, 4096            # 168 GFLOP: panels of 2816 and 2184 columns
