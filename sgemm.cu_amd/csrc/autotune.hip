@@ -100,19 +100,23 @@ int heuristic(int form, int grouped, int m, int n, int k, bool aligned) {
             // Configs 15 and 16 (256x256x32, one wave per SIMD and one
             // workgroup per CU; 15 gives config 5's bits, 16 adds a stream-K
             // tail): 8192^3 145.4 -> 150.1, 13312^3 146.3 -> 150.1 (16),
-            // 10240^3 145.8 -> 149.1 (16) TFLOP/s against config 9 on one box.
-            // Modelled times: a round of their tiles costs I k-tiles of
-            // kF256KtileUs each plus kF256RoundUs (prologue and epilogue of
-            // the whole chip at once), config 16's partial round what
-            // streamk_iters charges, and config 9 runs its padded tiles at
-            // 145 TFLOP/s.  Below k = 2048 the round cost makes config 9 as
-            // fast or faster (8192x8192x2048 142 vs 143, x512 117 vs 130); a
-            // tail of a few tiles cut into many pieces loses to it too (9216^3
-            // 144.5 vs 145.6, 7168^3 141.7 vs 144.9).
-            if (t128 >= 1024 && k % 32 == 0 && k >= 2048) {
+            // 10240^3 145.8 -> 149.1 (16), 8192x1024x8192 135 -> 141 (16)
+            // TFLOP/s on one box.  Modelled time t256 (microseconds): a round
+            // of their tiles costs I k-tiles of kF256KtileUs each plus
+            // kF256RoundUs (prologue and epilogue of the whole chip at once),
+            // config 16's partial round what streamk_iters charges.  It is
+            // weighed against the time the configs below would take, with 3 %
+            // in its favour: against measurements the model overstates it
+            // (4096^2x2048 141.3 vs config 9's 138.5, 8192x1024x8192 141.0 vs
+            // config 5's 135) more often than it understates it (8192^2x2048
+            // 141.8 vs 143.0, 9216^3 144.5 vs 145.6).  Below k = 2048 the
+            // round cost makes config 9 faster (8192^2x512 117 vs 130).
+            double t256 = 0.0;
+            int c256 = -1;
+            if (k % 32 == 0 && k >= 2048) {
                 const long long cus = device_cu_count();
-                const long long t256 = (long long)div_up(m, 256) * div_up(n, 256);
-                const long long iters = k / 32, full = t256 / cus, rem = t256 % cus;
+                const long long tiles = (long long)div_up(m, 256) * div_up(n, 256);
+                const long long iters = k / 32, full = tiles / cus, rem = tiles % cus;
                 const double round = (double)iters * kF256KtileUs + kF256RoundUs;
                 const double t15 = (double)(full + (rem > 0 ? 1 : 0)) * round;
                 double t16 = t15;
@@ -120,10 +124,16 @@ int heuristic(int form, int grouped, int m, int n, int k, bool aligned) {
                 if (rem > 0 && streamk_iters(rem, cus, iters, kF256SplitBytes,
                                              kF256StreamKPieceIters, &part) > 0)
                     t16 = (double)full * round + part * iters * kF256KtileUs + kF256RoundUs;
-                const double t9 = 2.0 * t128 * 16384.0 * k / 145.0e6;
-                if (std::min(t15, t16) < t9) return t16 < t15 ? 16 : 15;
+                c256 = t16 < t15 ? 16 : 15;
+                t256 = std::min(t15, t16) / 1.03;
             }
-            if (t128 >= 1024) return 9;
+            const double flops_us = 2.0 * m * n * (double)k / 1e6;  // MFLOP
+            if (t128 >= 1024) {
+                // config 9: its padded 128x128 tiles at 145 TFLOP/s (its tail
+                // split fills the last round)
+                const double t9 = 2.0 * t128 * 16384.0 * k / 145.0e6;
+                return (c256 >= 0 && t256 < t9) ? c256 : 9;
+            }
             // Short of one round of 1024 direct-loaded 128x128 tiles, with k
             // >= 1024 and from 180 tiles: every tile of a config takes the
             // same time, so with c tiles per CU on average the busiest CU
@@ -145,6 +155,10 @@ int heuristic(int form, int grouped, int m, int n, int k, bool aligned) {
                 const double e7 = 133.0 * fill(c7);
                 const long long s10 = std::llround(4.0 * cus / (double)t128);
                 const double e10 = s10 >= 2 ? 131.0 * fill((double)(t128 * s10) / cus) : 0.0;
+                const double e = std::max({e5, e7, e10});
+                // (only for long k here: at k = m the piece costs weigh more
+                // than the model gives them, 3072^3 124 vs config 7's 133)
+                if (c256 >= 0 && k >= 4096 && t256 < flops_us / e) return c256;
                 if (e5 >= e7 && e5 >= e10) return 5;
                 return e10 > e7 ? 10 : 7;
             }

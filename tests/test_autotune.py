@@ -108,15 +108,19 @@ def test_launch_config_follows_operand_alignment():
     assert sa.launch_config(F, 3968, 3968, 3968, base, 3968, base, 3968) == 5
     assert sa.launch_config(F, m, n, k, base, m + 1, base, k) == 9      # odd lda
     assert sa.launch_config(F, m, n, k, base + 4, m, base, k) == 9      # A 4 bytes off
-    # whole rounds of 256x256 tiles with k % 32 == 0: config 15; a partial
-    # last round (6144^3: 2.25 rounds) or odd columns: config 9
+    # whole rounds of 256x256 tiles with k % 32 == 0: config 15; odd
+    # columns: config 9; a partial last round (6144^3: 2.25 rounds) that
+    # stream-K spreads cheaply: config 16
     assert sa.launch_config(F, 8192, 8192, 8192, base, 8192, base, 8192) == 15
     assert sa.launch_config(F, 8192, 8192, 8192, base, 8193, base, 8192) == 9
-    assert sa.launch_config(F, 6144, 6144, 6144, base, 6144, base, 6144) == 9
+    assert sa.launch_config(F, 6144, 6144, 6144, base, 6144, base, 6144) == 16
     # a partial round that stream-K spreads cheaply: config 16 (13312^3: 144
     # tiles past ten rounds); a few tiles cut into many pieces: config 9
     assert sa.launch_config(F, 13312, 13312, 13312, base, 13312, base, 13312) == 16
-    assert sa.launch_config(F, 9216, 9216, 9216, base, 9216, base, 9216) == 9
+    assert sa.launch_config(F, 7168, 7168, 7168, base, 7168, base, 7168) == 9
+    # short of a round of 128x128 tiles with long k: stream-K over 256x256
+    # tiles where the model puts it ahead of the 64x64 / 128x128 fill model
+    assert sa.launch_config(F, 3072, 3072, 8192, base, 3072, base, 8192) == 16
     assert sa.launch_config(F, 8192, 8192, 127, base, 8193, base, 127) == 14
     assert sa.launch_config(F, m, n, 0, base, m, base, 1) == -1
     assert sa.launch_config(F, m, n, k, base, (1 << 21) + 4, base, k) == -1
